@@ -1,0 +1,208 @@
+/*
+ * tca_spmm.h -- C ABI of the MI355X (gfx950) row-balanced CSR x dense SpMM.
+ *
+ * Plain C: pointers, sizes, status codes.  Device pointers are HIP device
+ * addresses; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Every entry point returns a tca_status; tca_last_error() gives the detail of
+ * the last failure on the calling thread.  Nothing here allocates on the
+ * caller's behalf except where stated (plans, batches, tca_smtx_read).
+ *
+ * Each declaration names the reference interface it replaces
+ * (mlsys-lab-sogang/TCA-SpMM, paths relative to its root).
+ */
+#ifndef TCA_SPMM_H
+#define TCA_SPMM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCA_VERSION 100
+
+typedef enum tca_status {
+    TCA_OK = 0,
+    TCA_ERR_INVALID_ARG = 1,   /* bad shape, NULL pointer, misaligned buffer   */
+    TCA_ERR_UNSUPPORTED = 2,   /* valid but not implemented (dtype pair, N%8)   */
+    TCA_ERR_HIP = 3,           /* a HIP runtime call failed                     */
+    TCA_ERR_ALLOC = 4,         /* host or device allocation failed              */
+    TCA_ERR_IO = 5,            /* file could not be opened / written            */
+    TCA_ERR_FORMAT = 6,        /* malformed .smtx / inconsistent CSR            */
+    TCA_ERR_LIBRARY = 7        /* rocSPARSE / hipBLASLt reported an error       */
+} tca_status;
+
+typedef enum tca_dtype {
+    TCA_F16 = 0,   /* IEEE binary16 */
+    TCA_BF16 = 1,  /* bfloat16 */
+    TCA_E4M3 = 2,  /* OCP FP8 E4M3FN (gfx950 native) */
+    TCA_F32 = 3    /* binary32 (dense operand only; converted on load) */
+} tca_dtype;
+
+const char *tca_status_string(tca_status s);
+const char *tca_last_error(void);
+int tca_version(void);
+void tca_host_free(void *p);
+
+/* ------------------------------------------------------------------------ */
+/* .smtx I/O.  Replaces smtx_input_to_Dense (src/file_input.cu:135-193,      */
+/* decl src/file_input.cuh:8).  Returns the pattern as CSR (malloc'ed,        */
+/* release with tca_host_free) and rejects files the reference would overrun  */
+/* (token counts, non-monotone offsets, out-of-range columns).                */
+/* ------------------------------------------------------------------------ */
+tca_status tca_smtx_read(const char *path, int *n_rows, int *n_cols, int *nnz,
+                         int32_t **row_offset, int32_t **col_idx);
+tca_status tca_smtx_write(const char *path, int n_rows, int n_cols, int nnz,
+                          const int32_t *row_offset, const int32_t *col_idx);
+
+/* ------------------------------------------------------------------------ */
+/* Dense <-> CSR (fp16 bits).  Replaces convert_dense_to_csr /               */
+/* convert_csr_to_dense (src/miscalleneous.cu:29-65, :66-82).  With          */
+/* col_idx == NULL only counts (row_offset may also be NULL).                */
+/* ------------------------------------------------------------------------ */
+tca_status tca_dense_to_csr_f16(const uint16_t *dense, int n_rows, int n_cols, int32_t *row_offset,
+                                int32_t *col_idx, uint16_t *values, int capacity, int *nnz);
+tca_status tca_csr_to_dense_f16(int n_rows, int n_cols, const int32_t *row_offset, const int32_t *col_idx,
+                                const uint16_t *values, uint16_t *dense);
+
+/* compare_mat<float> (src/general.cuh:71-123): 1 = pass.  *n_fail counts
+ * failures up to the reference's early exit after the 10th. */
+int tca_compare_mat_f32(const float *gold, const float *cmp, int n, float tol, int *n_fail);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded synthetic inputs (counter-based SplitMix64 streams).  Replace       */
+/* prepare_cpu_matrix (src/general.cuh:140-174), which is unseeded.           */
+/* ------------------------------------------------------------------------ */
+typedef enum tca_pattern_kind {
+    TCA_PATTERN_BERNOULLI = 0,   /* each (r,c) kept with prob. density         */
+    TCA_PATTERN_LOGNORMAL = 1,   /* row nnz ~ lognormal(sigma=param), rescaled */
+    TCA_PATTERN_UNIFORM = 2      /* row nnz ~ U[(1-param),(1+param)] * mean    */
+} tca_pattern_kind;
+uint64_t tca_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);
+tca_status tca_gen_pattern(tca_pattern_kind kind, int n_rows, int n_cols, double density, double param,
+                           uint64_t seed, uint64_t stream, int32_t *row_offset /* n_rows+1 */,
+                           int32_t **col_idx /* malloc'ed */, int *nnz);
+tca_status tca_gen_normal_f32(uint64_t seed, uint64_t stream, size_t n, double mean, double std, float *out);
+tca_status tca_gen_uniform_f32(uint64_t seed, uint64_t stream, size_t n, double lo, double hi, float *out);
+tca_status tca_f32_to_f16_host(const float *in, size_t n, uint16_t *out);   /* round to nearest even */
+tca_status tca_f32_to_bf16_host(const float *in, size_t n, uint16_t *out);  /* round to nearest even */
+
+/* ------------------------------------------------------------------------ */
+/* Plan: the gfx950 panel tiling.  Replaces workload_manager                 */
+/* (src/our_spmm.cu:347-471), the reference's row->warp packing.  Rows are   */
+/* grouped `rows_per_wave` (16, 32 or 64) at a time; each group's nonzero     */
+/* columns are condensed and cut into 32-wide k-chunks; each (chunk, 16-row   */
+/* panel) lists its entries.  Values live outside the plan, in "tiled order" */
+/* (entry order); tca_pack_values converts from CSR order.  Built on the      */
+/* host from host CSR arrays; device tables go to the current HIP device.    */
+/* ------------------------------------------------------------------------ */
+typedef struct tca_plan tca_plan;
+typedef struct tca_plan_info {
+    int n_rows, n_cols, nnz;
+    int rows_per_wave;
+    int n_groups;
+    int n_chunks;
+    int max_group_chunks;
+    int transposed;            /* 1 for a plan made by tca_plan_create_transpose */
+    size_t device_bytes;
+    double build_ms;
+} tca_plan_info;
+
+tca_status tca_plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offset,
+                           const int32_t *col_idx, int rows_per_wave, tca_plan **plan);
+/* Same tables without device copies: for inspection (tca_plan_get_info,
+ * tca_plan_export) on machines without a GPU.  Launch entry points reject it. */
+tca_status tca_plan_create_host(int n_rows, int n_cols, int nnz, const int32_t *row_offset,
+                                const int32_t *col_idx, int rows_per_wave, tca_plan **plan);
+/* Plan of the transpose (n_cols x n_rows).  Its tiled values are read straight
+ * from the forward plan's tiled values (indirect), so one value array serves
+ * both directions. */
+tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_per_wave, tca_plan **plan_t);
+tca_status tca_plan_destroy(tca_plan *plan);
+tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *info);
+/* Host copies of the tables (for the host-rebuild check).  Sizes:
+ * grp_chunk_ptr[n_groups+1], chunk_cols[32*n_chunks], seg_ptr[n_chunks*rpw/16+1],
+ * entry_slot[nnz], entry_src[nnz], work_order[n_groups].  NULL skips a table. */
+tca_status tca_plan_export(const tca_plan *plan, int32_t *grp_chunk_ptr, int32_t *chunk_cols, int32_t *seg_ptr,
+                           uint16_t *entry_slot, int32_t *entry_src, int32_t *work_order);
+
+/* tiled[e] = csr[entry_src[e]] (and the inverse), elementwise on the device.
+ * dtype selects the element size (F16/BF16: 2 B, E4M3: 1 B, F32: 4 B).  For a
+ * transposed plan these use the forward plan's entry order. */
+tca_status tca_pack_values(const tca_plan *plan, tca_dtype dtype, const void *d_csr, void *d_tiled, void *stream);
+tca_status tca_unpack_values(const tca_plan *plan, tca_dtype dtype, const void *d_tiled, void *d_csr, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* SpMM: C[M x N] (f32, row-major, ldc) = A (plan + tiled values) * B.       */
+/* Replaces kernel_our_spmm_balanced (src/our_spmm.cu:290-346).              */
+/*   a_dtype in {F16, BF16, E4M3}; b_dtype == a_dtype, or F32 with a_dtype in */
+/*   {F16, BF16} (converted on load, round to nearest even).                 */
+/*   B row-major K x N with ldb; N % 8 == 0; B, C 16-byte aligned, ldb and    */
+/*   ldc multiples of 8 (elements).  Every row of C in [0, M) is written.    */
+/*   row_scale[M] / col_scale[N] (may be NULL) multiply the FP32 result:     */
+/*   C[r,n] = row_scale[r] * col_scale[n] * sum_k A[r,k] B[k,n] (E4M3).      */
+/*   tile_n in {16, 32, 64, 128}, or 0 to pick from the built-in table.      */
+/* ------------------------------------------------------------------------ */
+tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled,
+                    tca_dtype b_dtype, const void *d_B, int ldb, int N,
+                    float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
+                    int tile_n, void *stream);
+
+/* SDDMM on the plan's pattern: out_tiled[e] = sum_n G[row(e), n] * X[col(e), n]
+ * (the dW of Y = W X with dY = G).  G: M x N row-major (ldg), X: K x N (ldx),
+ * dtypes F16/BF16, or F32 (G) converted to F16 on load.  N % 32 == 0. */
+tca_status tca_sddmm(const tca_plan *plan, tca_dtype g_dtype, const void *d_G, int ldg,
+                     tca_dtype x_dtype, const void *d_X, int ldx, int N, float *d_out_tiled, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* FP8 E4M3 quantisation with FP32 scales (scale = amax / 448).               */
+/* ------------------------------------------------------------------------ */
+/* Per-row scales of a CSR matrix (values in CSR order, fp16 bits). */
+tca_status tca_quantize_rows_e4m3(int n_rows, const int32_t *d_row_offset, const uint16_t *d_values,
+                                  uint8_t *d_q, float *d_row_scale, void *stream);
+/* Per-column scales of a dense K x N fp16 matrix. */
+tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, uint8_t *d_q, int ldq,
+                                  float *d_col_scale, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched persistent launch over many layers (one kernel, capturable).      */
+/* ------------------------------------------------------------------------ */
+typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
+typedef struct tca_batch_item {
+    tca_op op;
+    const tca_plan *plan;
+    int tile_n;            /* SPMM: 16/32/64/128 (0 = table); SDDMM: ignored */
+    tca_dtype a_dtype;     /* SPMM: tiled values dtype; SDDMM: dtype of G    */
+    const void *a;         /* SPMM: tiled values;       SDDMM: G             */
+    int lda;               /* SDDMM: ldg                                     */
+    tca_dtype b_dtype;     /* dense operand dtype                            */
+    const void *b;         /* SPMM: B (K x N);          SDDMM: X (K x N)     */
+    int ldb;
+    int N;
+    float *out;            /* SPMM: C (M x N, ldc);     SDDMM: out_tiled     */
+    int ldc;
+} tca_batch_item;
+typedef struct tca_batch tca_batch;
+tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch);
+tca_status tca_batch_run(tca_batch *batch, void *stream);
+tca_status tca_batch_destroy(tca_batch *batch);
+/* Work units of a batch and the number of workgroups its launch uses. */
+tca_status tca_batch_get_info(const tca_batch *batch, int *n_units, int *n_workgroups);
+
+/* ------------------------------------------------------------------------ */
+/* Library baselines on device pointers (the reference times cuSPARSE and     */
+/* cuBLAS GemmEx: src/spmm_cusparse.cu:446-579, src/gemm_cublas.cu:3-111).    */
+/* ------------------------------------------------------------------------ */
+/* rocSPARSE CSR SpMM, f16 A/B -> f32 C.  ms_per_iter = mean over n_iter after
+ * warm-up (pass n_iter = 0 for a single untimed call). */
+tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row_offset, const int32_t *d_col_idx,
+                              const uint16_t *d_values, const uint16_t *d_B, float *d_C, int n_iter,
+                              double *ms_per_iter);
+/* hipBLASLt dense GEMM C = A_dense * B, f16 in, f32 out (row-major). */
+tca_status tca_hipblaslt_gemm(int M, int N, int K, const uint16_t *d_A, const uint16_t *d_B, float *d_C,
+                              int n_iter, double *ms_per_iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCA_SPMM_H */

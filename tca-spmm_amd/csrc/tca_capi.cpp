@@ -1,0 +1,255 @@
+// C ABI: plan lifetime, value packing, SpMM / SDDMM / quantisation entry
+// points.  Argument checking happens here; kernels assume valid arguments.
+#include "tca_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+using tca::fail;
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+tca_status upload(tca_plan *p)
+{
+    const tca::PlanTables &t = p->host;
+    const int ppg = t.rpw / 16;
+    const size_t sz[6] = {
+        sizeof(int32_t) * (size_t)(t.n_groups + 1),
+        sizeof(int32_t) * (size_t)t.n_chunks * 32,
+        sizeof(int32_t) * ((size_t)t.n_chunks * ppg + 1),
+        sizeof(uint16_t) * (size_t)t.nnz,
+        sizeof(int32_t) * (size_t)t.nnz,
+        sizeof(int32_t) * (size_t)t.n_groups,
+    };
+    size_t off[6], total = 0;
+    for (int i = 0; i < 6; i++) {
+        off[i] = total;
+        total = align_up(total + std::max<size_t>(sz[i], 4), 256);
+    }
+    TCA_HIP_TRY(hipGetDevice(&p->device));
+    void *base = nullptr;
+    if (hipMalloc(&base, total) != hipSuccess) return fail(TCA_ERR_ALLOC, "plan: device allocation failed");
+    p->d_base = base;
+    p->d_bytes = total;
+    char *b = static_cast<char *>(base);
+    const void *src[6] = {t.grp_chunk_ptr.data(), t.chunk_cols.data(), t.seg_ptr.data(),
+                          t.entry_slot.data(),    t.entry_src.data(),  t.work_order.data()};
+    for (int i = 0; i < 6; i++)
+        if (sz[i]) TCA_HIP_TRY(hipMemcpy(b + off[i], src[i], sz[i], hipMemcpyHostToDevice));
+    p->d_grp_chunk_ptr = reinterpret_cast<const int32_t *>(b + off[0]);
+    p->d_chunk_cols = reinterpret_cast<const int32_t *>(b + off[1]);
+    p->d_seg_ptr = reinterpret_cast<const int32_t *>(b + off[2]);
+    p->d_entry_slot = reinterpret_cast<const uint16_t *>(b + off[3]);
+    p->d_entry_src = reinterpret_cast<const int32_t *>(b + off[4]);
+    p->d_work_order = reinterpret_cast<const int32_t *>(b + off[5]);
+    return TCA_OK;
+}
+
+int dtype_bytes(tca_dtype d)
+{
+    switch (d) {
+    case TCA_F16:
+    case TCA_BF16: return 2;
+    case TCA_E4M3: return 1;
+    case TCA_F32: return 4;
+    }
+    return 0;
+}
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+}  // namespace
+
+namespace {
+
+tca_status plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx,
+                       int rows_per_wave, tca_plan **plan, bool device)
+{
+    if (!plan) return fail(TCA_ERR_INVALID_ARG, "tca_plan_create: NULL plan");
+    *plan = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    tca_plan *p = new (std::nothrow) tca_plan();
+    if (!p) return fail(TCA_ERR_ALLOC, "tca_plan_create: host allocation");
+    tca_status s = tca::build_tables(n_rows, n_cols, nnz, row_offset, col_idx, rows_per_wave, p->host);
+    if (s != TCA_OK) { delete p; return s; }
+    p->csr_row_offset.assign(row_offset, row_offset + n_rows + 1);
+    p->csr_col_idx.assign(col_idx, col_idx + nnz);
+    p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (device) {
+        s = upload(p);
+        if (s != TCA_OK) { tca_plan_destroy(p); return s; }
+    }
+    *plan = p;
+    return TCA_OK;
+}
+
+}  // namespace
+
+extern "C" tca_status tca_plan_create_host(int n_rows, int n_cols, int nnz, const int32_t *row_offset,
+                                           const int32_t *col_idx, int rows_per_wave, tca_plan **plan)
+{
+    return plan_create(n_rows, n_cols, nnz, row_offset, col_idx, rows_per_wave, plan, false);
+}
+
+extern "C" tca_status tca_plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx,
+                                      int rows_per_wave, tca_plan **plan)
+{
+    return plan_create(n_rows, n_cols, nnz, row_offset, col_idx, rows_per_wave, plan, true);
+}
+
+extern "C" tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_per_wave, tca_plan **plan_t)
+{
+    if (!fwd || !plan_t) return fail(TCA_ERR_INVALID_ARG, "tca_plan_create_transpose: NULL argument");
+    if (fwd->transposed) return fail(TCA_ERR_INVALID_ARG, "tca_plan_create_transpose: plan is already a transpose");
+    *plan_t = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    const int M = fwd->host.n_rows, K = fwd->host.n_cols, nnz = fwd->host.nnz;
+    // CSC of the forward CSR: rows of the transpose, with the CSR index of each entry
+    std::vector<int32_t> rpT(K + 1, 0), cT(nnz), perm(nnz);
+    const std::vector<int32_t> &rp = fwd->csr_row_offset, &ci = fwd->csr_col_idx;
+    for (int e = 0; e < nnz; e++) rpT[ci[e] + 1]++;
+    for (int j = 0; j < K; j++) rpT[j + 1] += rpT[j];
+    std::vector<int32_t> fill(rpT.begin(), rpT.end() - 1);
+    for (int r = 0; r < M; r++)
+        for (int e = rp[r]; e < rp[r + 1]; e++) {
+            const int d = fill[ci[e]]++;
+            cT[d] = r;
+            perm[d] = e;
+        }
+    tca_plan *p = new (std::nothrow) tca_plan();
+    if (!p) return fail(TCA_ERR_ALLOC, "tca_plan_create_transpose: host allocation");
+    tca_status s = tca::build_tables(K, M, nnz, rpT.data(), cT.data(), rows_per_wave, p->host);
+    if (s != TCA_OK) { delete p; return s; }
+    // entry_src: transpose CSR index -> forward CSR index -> forward tiled index
+    std::vector<int32_t> fwd_tiled_of_csr(nnz);
+    for (int e = 0; e < nnz; e++) fwd_tiled_of_csr[fwd->host.entry_src[e]] = e;
+    for (int e = 0; e < nnz; e++) p->host.entry_src[e] = fwd_tiled_of_csr[perm[p->host.entry_src[e]]];
+    p->transposed = 1;
+    p->csr_row_offset = std::move(rpT);
+    p->csr_col_idx = std::move(cT);
+    p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (fwd->d_base) {  // a host-only forward plan gives a host-only transpose
+        s = upload(p);
+        if (s != TCA_OK) { tca_plan_destroy(p); return s; }
+    }
+    *plan_t = p;
+    return TCA_OK;
+}
+
+extern "C" tca_status tca_plan_destroy(tca_plan *plan)
+{
+    if (!plan) return TCA_OK;
+    int cur = 0;
+    bool switched = false;
+    if ((plan->d_base || plan->d_sddmm_units) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
+        switched = hipSetDevice(plan->device) == hipSuccess;
+    if (plan->d_base) (void)hipFree(plan->d_base);
+    if (plan->d_sddmm_units) (void)hipFree(plan->d_sddmm_units);
+    if (switched) (void)hipSetDevice(cur);
+    delete plan;
+    return TCA_OK;
+}
+
+extern "C" tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *info)
+{
+    if (!plan || !info) return fail(TCA_ERR_INVALID_ARG, "tca_plan_get_info: NULL argument");
+    const tca::PlanTables &t = plan->host;
+    info->n_rows = t.n_rows;
+    info->n_cols = t.n_cols;
+    info->nnz = t.nnz;
+    info->rows_per_wave = t.rpw;
+    info->n_groups = t.n_groups;
+    info->n_chunks = t.n_chunks;
+    info->max_group_chunks = t.max_group_chunks;
+    info->transposed = plan->transposed;
+    info->device_bytes = plan->d_bytes;
+    info->build_ms = plan->build_ms;
+    return TCA_OK;
+}
+
+extern "C" tca_status tca_plan_export(const tca_plan *plan, int32_t *grp_chunk_ptr, int32_t *chunk_cols, int32_t *seg_ptr,
+                                      uint16_t *entry_slot, int32_t *entry_src, int32_t *work_order)
+{
+    if (!plan) return fail(TCA_ERR_INVALID_ARG, "tca_plan_export: NULL plan");
+    const tca::PlanTables &t = plan->host;
+    auto cp = [](auto *dst, const auto &v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(grp_chunk_ptr, t.grp_chunk_ptr);
+    cp(chunk_cols, t.chunk_cols);
+    cp(seg_ptr, t.seg_ptr);
+    cp(entry_slot, t.entry_slot);
+    cp(entry_src, t.entry_src);
+    cp(work_order, t.work_order);
+    return TCA_OK;
+}
+
+extern "C" tca_status tca_pack_values(const tca_plan *plan, tca_dtype dtype, const void *d_csr, void *d_tiled, void *stream)
+{
+    if (!plan || (plan->host.nnz > 0 && (!d_csr || !d_tiled))) return fail(TCA_ERR_INVALID_ARG, "tca_pack_values: NULL argument");
+    if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_pack_values: host-only plan");
+    if (plan->transposed) return fail(TCA_ERR_INVALID_ARG, "tca_pack_values: use the forward plan");
+    const int eb = dtype_bytes(dtype);
+    if (!eb) return fail(TCA_ERR_INVALID_ARG, "tca_pack_values: dtype");
+    return tca::launch_gather(plan->d_entry_src, plan->host.nnz, eb, d_csr, d_tiled, false, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_unpack_values(const tca_plan *plan, tca_dtype dtype, const void *d_tiled, void *d_csr, void *stream)
+{
+    if (!plan || (plan->host.nnz > 0 && (!d_csr || !d_tiled))) return fail(TCA_ERR_INVALID_ARG, "tca_unpack_values: NULL argument");
+    if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_unpack_values: host-only plan");
+    if (plan->transposed) return fail(TCA_ERR_INVALID_ARG, "tca_unpack_values: use the forward plan");
+    const int eb = dtype_bytes(dtype);
+    if (!eb) return fail(TCA_ERR_INVALID_ARG, "tca_unpack_values: dtype");
+    return tca::launch_gather(plan->d_entry_src, plan->host.nnz, eb, d_tiled, d_csr, true, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled, tca_dtype b_dtype,
+                               const void *d_B, int ldb, int N, float *d_C, int ldc, const float *d_row_scale,
+                               const float *d_col_scale, int tile_n, void *stream)
+{
+    if (!plan) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: NULL plan");
+    if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: host-only plan");
+    if (N <= 0) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: N must be positive");
+    if (N % 8) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: N must be a multiple of 8");
+    if (ldb < N || ldc < N || ldb % 8 || ldc % 4) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: ldb/ldc too small or misaligned");
+    if (!d_B || !d_C || (plan->host.nnz > 0 && !d_a_tiled)) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: NULL buffer");
+    if (!aligned(d_B, 16) || !aligned(d_C, 16)) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: B and C must be 16-byte aligned");
+    if (tile_n != 0 && tile_n != 16 && tile_n != 32 && tile_n != 64 && tile_n != 128)
+        return fail(TCA_ERR_INVALID_ARG, "tca_spmm: tile_n must be 0, 16, 32, 64 or 128");
+    if (b_dtype == TCA_F32 && a_dtype == TCA_E4M3) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: quantise B to E4M3 first");
+    if (b_dtype != a_dtype && b_dtype != TCA_F32) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: B dtype must equal A dtype or be F32");
+    return tca::launch_spmm(plan, a_dtype, d_a_tiled, b_dtype, d_B, ldb, N, d_C, ldc, d_row_scale, d_col_scale, tile_n,
+                            static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_sddmm(const tca_plan *plan, tca_dtype g_dtype, const void *d_G, int ldg, tca_dtype x_dtype,
+                                const void *d_X, int ldx, int N, float *d_out_tiled, void *stream)
+{
+    if (!plan) return fail(TCA_ERR_INVALID_ARG, "tca_sddmm: NULL plan");
+    if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_sddmm: host-only plan");
+    if (N <= 0 || N % 32) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: N must be a positive multiple of 32");
+    if (ldg < N || ldx < N || ldg % 8 || ldx % 8) return fail(TCA_ERR_INVALID_ARG, "tca_sddmm: ldg/ldx too small or misaligned");
+    if (!d_G || !d_X || (plan->host.nnz > 0 && !d_out_tiled)) return fail(TCA_ERR_INVALID_ARG, "tca_sddmm: NULL buffer");
+    if (!aligned(d_G, 16) || !aligned(d_X, 16)) return fail(TCA_ERR_INVALID_ARG, "tca_sddmm: G and X must be 16-byte aligned");
+    return tca::launch_sddmm(plan, g_dtype, d_G, ldg, x_dtype, d_X, ldx, N, d_out_tiled, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_quantize_rows_e4m3(int n_rows, const int32_t *d_row_offset, const uint16_t *d_values, uint8_t *d_q,
+                                             float *d_row_scale, void *stream)
+{
+    if (n_rows < 0 || (n_rows > 0 && (!d_row_offset || !d_row_scale))) return fail(TCA_ERR_INVALID_ARG, "tca_quantize_rows_e4m3: bad argument");
+    return tca::launch_quantize_rows(n_rows, d_row_offset, d_values, d_q, d_row_scale, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, uint8_t *d_q, int ldq,
+                                             float *d_col_scale, void *stream)
+{
+    if (K < 0 || N < 0 || ldb < N || ldq < N || (K > 0 && N > 0 && (!d_B || !d_q || !d_col_scale)))
+        return fail(TCA_ERR_INVALID_ARG, "tca_quantize_cols_e4m3: bad argument");
+    return tca::launch_quantize_cols(K, N, d_B, ldb, d_q, ldq, d_col_scale, static_cast<hipStream_t>(stream));
+}

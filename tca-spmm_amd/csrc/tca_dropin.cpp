@@ -1,0 +1,263 @@
+// C++ drop-in API (include/tca_spmm.hpp): the reference's host entry points
+// with the same names, arguments and side outputs, implemented on the C ABI.
+#include "tca_internal.hpp"
+#include "../../include/tca_spmm.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iterator>
+#include <sstream>
+#include <type_traits>
+
+namespace {
+
+[[noreturn]] void die(const char *what, tca_status s)
+{
+    std::fprintf(stderr, "TCA error in %s: %s (%s)\n", what, tca_status_string(s), tca_last_error());
+    std::exit(EXIT_FAILURE);
+}
+
+#define TCA_CHECK(expr)                         \
+    do {                                        \
+        tca_status _s = (expr);                 \
+        if (_s != TCA_OK) die(#expr, _s);       \
+    } while (0)
+
+#define HIP_CHECK(expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) {                                                                          \
+            std::fprintf(stderr, "HIP error at [%s : %d] %d %s\n", __FILE__, __LINE__, (int)_e, hipGetErrorString(_e)); \
+            std::exit(EXIT_FAILURE);                                                                     \
+        }                                                                                                \
+    } while (0)
+
+inline uint16_t hbits(half h)
+{
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+// rows_per_wave / tile_n used by the drop-in when nothing else is known
+void default_config(int M, int K, int nnz, int N, int *rpw, int *tile_n)
+{
+    const double density = (M > 0 && K > 0) ? (double)nnz / ((double)M * K) : 0.0;
+    *rpw = density > 0.05 ? 32 : 16;
+    *tile_n = N >= 128 ? 64 : (N >= 32 ? 32 : 16);
+}
+
+}  // namespace
+
+void warm_up_gpu()
+{
+    TCA_CHECK(tca::launch_warm_up(nullptr));
+    HIP_CHECK(hipDeviceSynchronize());
+}
+
+// src/file_input.cu:135-193: dense fp16 with 1.0 at each pattern position
+void smtx_input_to_Dense(std::ifstream &fin, half **mat, char *filename, int *nRow, int *nCol, int *nnz)
+{
+    (void)filename;
+    if (!fin.is_open()) {
+        std::fprintf(stderr, "smtx_input_to_Dense: input stream is not open\n");
+        std::exit(EXIT_FAILURE);
+    }
+    std::string text((std::istreambuf_iterator<char>(fin)), std::istreambuf_iterator<char>());
+    int32_t *rp = nullptr, *ci = nullptr;
+    TCA_CHECK(tca::smtx_parse(text, nRow, nCol, nnz, &rp, &ci));
+    const size_t total = (size_t)(*nRow) * (size_t)(*nCol);
+    uint16_t *d = static_cast<uint16_t *>(std::calloc(total ? total : 1, sizeof(uint16_t)));
+    if (!d) die("smtx_input_to_Dense", TCA_ERR_ALLOC);
+    for (int r = 0; r < *nRow; r++)
+        for (int e = rp[r]; e < rp[r + 1]; e++) d[(size_t)r * (*nCol) + ci[e]] = 0x3C00;  // 1.0
+    std::free(rp);
+    std::free(ci);
+    *mat = reinterpret_cast<half *>(d);
+}
+
+// src/miscalleneous.cu:29-65
+void convert_dense_to_csr(half *mat, CSR **descr, int nRow, int nCol)
+{
+    const uint16_t *d = reinterpret_cast<const uint16_t *>(mat);
+    int nnz = 0;
+    TCA_CHECK(tca_dense_to_csr_f16(d, nRow, nCol, nullptr, nullptr, nullptr, 0, &nnz));
+    CSR *c = static_cast<CSR *>(std::malloc(sizeof(CSR)));
+    c->nRow = nRow;
+    c->nCol = nCol;
+    c->nnz = nnz;
+    c->row_offset = static_cast<int *>(std::malloc(sizeof(int) * ((size_t)nRow + 1)));
+    c->col_idx = static_cast<int *>(std::malloc(sizeof(int) * ((size_t)nnz + 1)));
+    c->value = static_cast<half *>(std::malloc(sizeof(half) * ((size_t)nnz + 1)));
+    TCA_CHECK(tca_dense_to_csr_f16(d, nRow, nCol, c->row_offset, c->col_idx, reinterpret_cast<uint16_t *>(c->value),
+                                   nnz, &nnz));
+    *descr = c;
+}
+
+// src/miscalleneous.cu:66-82
+void convert_csr_to_dense(CSR *descr, half **mat)
+{
+    const size_t total = (size_t)descr->nRow * (size_t)descr->nCol;
+    uint16_t *d = static_cast<uint16_t *>(std::malloc(sizeof(uint16_t) * (total ? total : 1)));
+    TCA_CHECK(tca_csr_to_dense_f16(descr->nRow, descr->nCol, descr->row_offset, descr->col_idx,
+                                   reinterpret_cast<const uint16_t *>(descr->value), d));
+    *mat = reinterpret_cast<half *>(d);
+}
+
+// src/miscalleneous.cu:83-89
+void destroy_csr(CSR *descr)
+{
+    std::free(descr->col_idx);
+    std::free(descr->row_offset);
+    std::free(descr->value);
+    std::free(descr);
+}
+
+// src/general.cuh:71-123 (the discrepancy lines are printed like the reference)
+template <typename T>
+bool compare_mat(T *gold, T *comparing, int num_elem, float tolerance)
+{
+    bool success = true;
+    int count = 0;
+    for (int i = 0; i < num_elem; i++) {
+        float v1, v2;
+        if constexpr (std::is_same<T, half>::value) {
+            v1 = __half2float(gold[i]);
+            v2 = __half2float(comparing[i]);
+        } else {
+            v1 = (float)gold[i];
+            v2 = (float)comparing[i];
+        }
+        const float disc = std::fabs(v1 - v2);
+        const float rel = std::fabs(disc / v1);
+        if (rel >= tolerance && disc >= tolerance) {
+            success = false;
+            std::printf("Discrepancy occured in %dth element :: %f vs %f ...abs_disc : %f, rel_disc : %f %%\n", i, v1, v2,
+                        disc, rel * 100.0);
+            if (++count >= 10) return success;
+        }
+    }
+    return success;
+}
+template bool compare_mat<float>(float *, float *, int, float);
+template bool compare_mat<half>(half *, half *, int, float);
+
+// src/our_spmm.cu:472-588: host CSR in, host C in/out, mean ms per launch out.
+void our_spmm_balanced(CSR *A_csr, half *B, float *C, const int M, const int N, const int K, int n_iter, double *elapsed)
+{
+    warm_up_gpu();
+    if (A_csr->nRow != M || A_csr->nCol != K) {
+        std::fprintf(stderr, "our_spmm_balanced: CSR is %dx%d, expected %dx%d\n", A_csr->nRow, A_csr->nCol, M, K);
+        std::exit(EXIT_FAILURE);
+    }
+    const int nnz = A_csr->nnz;
+    uint16_t *dA = nullptr, *dAt = nullptr, *dB = nullptr;
+    float *dC = nullptr;
+    HIP_CHECK(hipMalloc(&dA, sizeof(uint16_t) * ((size_t)nnz + 1)));
+    HIP_CHECK(hipMalloc(&dAt, sizeof(uint16_t) * ((size_t)nnz + 1)));
+    HIP_CHECK(hipMalloc(&dB, sizeof(uint16_t) * (size_t)K * N));
+    HIP_CHECK(hipMalloc(&dC, sizeof(float) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(dA, A_csr->value, sizeof(uint16_t) * (size_t)nnz, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dB, B, sizeof(uint16_t) * (size_t)K * N, hipMemcpyHostToDevice));
+
+    int rpw, tile_n;
+    default_config(M, K, nnz, N, &rpw, &tile_n);
+    auto t0 = std::chrono::high_resolution_clock::now();
+    tca_plan *plan = nullptr;
+    TCA_CHECK(tca_plan_create(M, K, nnz, A_csr->row_offset, A_csr->col_idx, rpw, &plan));
+    TCA_CHECK(tca_pack_values(plan, TCA_F16, dA, dAt, nullptr));
+    HIP_CHECK(hipDeviceSynchronize());
+    auto t1 = std::chrono::high_resolution_clock::now();
+    std::printf("preprocess %fms taken\n", std::chrono::duration<float, std::milli>(t1 - t0).count());
+    tca_plan_info info;
+    TCA_CHECK(tca_plan_get_info(plan, &info));
+    const int n_tiles = (N + tile_n - 1) / tile_n;
+    const size_t lds = (size_t)tile_n * 32 * 2 + (size_t)(rpw / 16) * 1024;
+    std::printf("num_block: %d blockdim: %d shm: %fKB\n", info.n_groups * n_tiles, 64, (float)lds / 1024.0f);
+
+    for (int i = 0; i < 5; i++)
+        TCA_CHECK(tca_spmm(plan, TCA_F16, dAt, TCA_F16, dB, N, N, dC, N, nullptr, nullptr, tile_n, nullptr));
+    HIP_CHECK(hipDeviceSynchronize());
+    hipEvent_t ev0, ev1;
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+    const int iters = n_iter > 0 ? n_iter : 1;
+    HIP_CHECK(hipEventRecord(ev0, nullptr));
+    for (int i = 0; i < iters; i++)
+        TCA_CHECK(tca_spmm(plan, TCA_F16, dAt, TCA_F16, dB, N, N, dC, N, nullptr, nullptr, tile_n, nullptr));
+    HIP_CHECK(hipEventRecord(ev1, nullptr));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    if (elapsed) *elapsed = (double)ms / iters;
+
+    // rows without nonzeros keep the caller's values (the reference never writes them)
+    float *hC = static_cast<float *>(std::malloc(sizeof(float) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(hC, dC, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+    for (int r = 0; r < M; r++)
+        if (A_csr->row_offset[r + 1] > A_csr->row_offset[r])
+            std::memcpy(C + (size_t)r * N, hC + (size_t)r * N, sizeof(float) * (size_t)N);
+    std::free(hC);
+    HIP_CHECK(hipEventDestroy(ev0));
+    HIP_CHECK(hipEventDestroy(ev1));
+    tca_plan_destroy(plan);
+    HIP_CHECK(hipFree(dA));
+    HIP_CHECK(hipFree(dAt));
+    HIP_CHECK(hipFree(dB));
+    HIP_CHECK(hipFree(dC));
+}
+
+// src/spmm_cusparse.cu:446-579 on rocSPARSE (dense A converted to CSR inside, untimed)
+void perform_rocSPARSE_SpMM(half *A, half *B, float *C, int M, int N, int K, int num_iter, double *elapsed)
+{
+    warm_up_gpu();
+    CSR *csr;
+    convert_dense_to_csr(A, &csr, M, K);
+    int32_t *drp, *dci;
+    uint16_t *dv, *dB;
+    float *dC;
+    HIP_CHECK(hipMalloc(&drp, sizeof(int32_t) * ((size_t)M + 1)));
+    HIP_CHECK(hipMalloc(&dci, sizeof(int32_t) * ((size_t)csr->nnz + 1)));
+    HIP_CHECK(hipMalloc(&dv, sizeof(uint16_t) * ((size_t)csr->nnz + 1)));
+    HIP_CHECK(hipMalloc(&dB, sizeof(uint16_t) * (size_t)K * N));
+    HIP_CHECK(hipMalloc(&dC, sizeof(float) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(drp, csr->row_offset, sizeof(int32_t) * ((size_t)M + 1), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dci, csr->col_idx, sizeof(int32_t) * (size_t)csr->nnz, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dv, csr->value, sizeof(uint16_t) * (size_t)csr->nnz, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dB, B, sizeof(uint16_t) * (size_t)K * N, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dC, C, sizeof(float) * (size_t)M * N, hipMemcpyHostToDevice));
+    double ms = 0.0;
+    TCA_CHECK(tca_rocsparse_spmm(M, N, K, csr->nnz, drp, dci, dv, dB, dC, num_iter > 0 ? num_iter : 1, &ms));
+    if (elapsed) *elapsed = ms;
+    HIP_CHECK(hipMemcpy(C, dC, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipFree(drp));
+    HIP_CHECK(hipFree(dci));
+    HIP_CHECK(hipFree(dv));
+    HIP_CHECK(hipFree(dB));
+    HIP_CHECK(hipFree(dC));
+    destroy_csr(csr);
+}
+
+// src/gemm_cublas.cu:3-111 on hipBLASLt
+void perform_hipBLASLt_GemmEx(half *A, half *B, float *C, int M, int N, int K, int n_iter, double *elapsed)
+{
+    warm_up_gpu();
+    uint16_t *dA, *dB;
+    float *dC;
+    HIP_CHECK(hipMalloc(&dA, sizeof(uint16_t) * (size_t)M * K));
+    HIP_CHECK(hipMalloc(&dB, sizeof(uint16_t) * (size_t)K * N));
+    HIP_CHECK(hipMalloc(&dC, sizeof(float) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(dA, A, sizeof(uint16_t) * (size_t)M * K, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dB, B, sizeof(uint16_t) * (size_t)K * N, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dC, C, sizeof(float) * (size_t)M * N, hipMemcpyHostToDevice));
+    double ms = 0.0;
+    TCA_CHECK(tca_hipblaslt_gemm(M, N, K, dA, dB, dC, n_iter > 0 ? n_iter : 1, &ms));
+    if (elapsed) *elapsed = ms;
+    HIP_CHECK(hipMemcpy(C, dC, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipFree(dA));
+    HIP_CHECK(hipFree(dB));
+    HIP_CHECK(hipFree(dC));
+}

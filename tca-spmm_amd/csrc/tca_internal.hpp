@@ -1,0 +1,83 @@
+// Internal declarations shared by the host code, the kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include "../../include/tca_spmm.h"
+
+namespace tca {
+
+// Thread-local error detail behind tca_last_error().
+void set_error(const std::string &msg);
+tca_status fail(tca_status s, const std::string &msg);
+
+#define TCA_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::tca::fail(TCA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Panel tiling of one sparse matrix (see tca_plan_create).
+struct PlanTables {
+    int n_rows = 0, n_cols = 0, nnz = 0, rpw = 16;
+    int n_groups = 0, n_chunks = 0, max_group_chunks = 0;
+    std::vector<int32_t> grp_chunk_ptr;  // n_groups + 1
+    std::vector<int32_t> chunk_cols;     // 32 * n_chunks, -1 padded
+    std::vector<int32_t> seg_ptr;        // n_chunks * ppg + 1
+    std::vector<uint16_t> entry_slot;    // nnz: (r16 << 5) | k
+    std::vector<int32_t> entry_src;      // nnz: CSR index (or forward tiled index)
+    std::vector<int32_t> work_order;     // n_groups, heavy first
+};
+
+// Parses .smtx text into a malloc'ed CSR pattern.
+tca_status smtx_parse(const std::string &text, int *n_rows, int *n_cols, int *nnz, int32_t **row_offset,
+                      int32_t **col_idx);
+
+// Builds the tables from a CSR pattern (sorted, distinct columns per row).
+tca_status build_tables(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx,
+                        int rpw, PlanTables &t);
+
+}  // namespace tca
+
+struct tca_plan {
+    tca::PlanTables host;
+    int transposed = 0;
+    int device = 0;
+    double build_ms = 0.0;
+    // device tables (one allocation)
+    void *d_base = nullptr;
+    size_t d_bytes = 0;
+    const int32_t *d_grp_chunk_ptr = nullptr;
+    const int32_t *d_chunk_cols = nullptr;
+    const int32_t *d_seg_ptr = nullptr;
+    const uint16_t *d_entry_slot = nullptr;
+    const int32_t *d_entry_src = nullptr;
+    const int32_t *d_work_order = nullptr;
+    // SDDMM work units (group, first chunk), built on first use
+    void *d_sddmm_units = nullptr;
+    int n_sddmm_units = 0;
+    // CSR pattern kept for building the transpose
+    std::vector<int32_t> csr_row_offset, csr_col_idx;
+};
+
+namespace tca {
+
+// Kernel launchers (tca_kernels.hip).
+tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_dtype b_dtype, const void *B,
+                       int ldb, int N, float *C, int ldc, const float *row_scale, const float *col_scale,
+                       int tile_n, hipStream_t stream);
+tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype,
+                        const void *X, int ldx, int N, float *out, hipStream_t stream);
+tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *src, void *dst, bool scatter,
+                         hipStream_t stream);
+tca_status launch_quantize_rows(int n_rows, const int32_t *row_offset, const uint16_t *vals, uint8_t *q,
+                                float *scale, hipStream_t stream);
+tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq, float *scale,
+                                hipStream_t stream);
+int default_tile_n(const tca_plan *p, int N);
+tca_status launch_warm_up(hipStream_t stream);
+
+}  // namespace tca

@@ -1,0 +1,652 @@
+// gfx950 kernels: panel-tiled SpMM (MFMA 16x16x32, FP32 accumulate), SDDMM,
+// value gather/scatter and FP8 E4M3 quantisation.
+//
+// Data layout (built by build_tables):
+//   rows are grouped RPW (16/32/64) at a time; group g's nonzero columns are
+//   condensed and cut into 32-wide chunks (chunk_cols, -1 padded); each
+//   (chunk, 16-row panel) segment lists entries as slot = (r16 << 5) | k with
+//   the value at the same index of the tiled value array.
+//
+// One wave (64 lanes, one workgroup) owns one (group, N-tile of NT columns).
+// Per chunk it gathers the 32 condensed B rows x NT columns into LDS in
+// k-contiguous order (the MFMA B operand wants 8 consecutive k per lane),
+// scatters the panel entries into a zeroed 16x32 LDS A tile, and issues
+// PPG x NT/16 MFMAs.  Reference hot loop: src/our_spmm.cu:84-155 (gather,
+// transpose into shared memory, mma.sync m16n8k8) -- here the operands are
+// laid out for v_mfma_f32_16x16x32_{f16,bf16,fp8_fp8} on 64-wide waves.
+#include "tca_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+
+namespace tca {
+namespace {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// MFMA element traits.  An "octet" is 8 consecutive-k elements of one operand
+// lane: 16 bytes for f16/bf16, 8 bytes for fp8.
+// ---------------------------------------------------------------------------
+template <tca_dtype DT> struct Elt;
+template <> struct Elt<TCA_F16> {
+    static constexpr int ES = 2;
+    using Oct = uint4;
+    __device__ static f32x4 mma(Oct a, Oct b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    }
+};
+template <> struct Elt<TCA_BF16> {
+    static constexpr int ES = 2;
+    using Oct = uint4;
+    __device__ static f32x4 mma(Oct a, Oct b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+};
+template <> struct Elt<TCA_E4M3> {
+    static constexpr int ES = 1;
+    using Oct = uint2;
+    __device__ static f32x4 mma(Oct a, Oct b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(__builtin_bit_cast(long, a), __builtin_bit_cast(long, b), c, 0, 0, 0);
+    }
+};
+
+template <class Oct> __device__ inline Oct zero_oct();
+template <> __device__ inline uint4 zero_oct<uint4>() { return make_uint4(0, 0, 0, 0); }
+template <> __device__ inline uint2 zero_oct<uint2>() { return make_uint2(0, 0); }
+
+// element e (0..7) of an octet, as raw bits
+__device__ inline uint32_t oct_elem(const uint4 &o, int e)
+{
+    const uint32_t w = e < 2 ? o.x : e < 4 ? o.y : e < 6 ? o.z : o.w;
+    return (w >> ((e & 1) * 16)) & 0xffffu;
+}
+__device__ inline uint32_t oct_elem(const uint2 &o, int e)
+{
+    const uint32_t w = e < 4 ? o.x : o.y;
+    return (w >> ((e & 3) * 8)) & 0xffu;
+}
+
+__device__ inline uint32_t f32_to_f16_bits(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f); }
+__device__ inline uint32_t f32_to_bf16_bits(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// Load 8 consecutive elements of dense row `row` (columns n..n+7) and convert
+// them to the MFMA element type AT.
+template <tca_dtype AT, tca_dtype BT> struct RowLoad;
+template <tca_dtype AT> struct RowLoad<AT, AT> {
+    using Oct = typename Elt<AT>::Oct;
+    __device__ static Oct load(const void *B, long off)
+    {
+        using E = typename std::conditional<Elt<AT>::ES == 2, uint16_t, uint8_t>::type;
+        return *reinterpret_cast<const Oct *>(static_cast<const E *>(B) + off);
+    }
+};
+template <> struct RowLoad<TCA_F16, TCA_F32> {
+    __device__ static uint4 load(const void *B, long off)
+    {
+        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
+        const float4 a = p[0], b = p[1];
+        return make_uint4(f32_to_f16_bits(a.x) | f32_to_f16_bits(a.y) << 16, f32_to_f16_bits(a.z) | f32_to_f16_bits(a.w) << 16,
+                          f32_to_f16_bits(b.x) | f32_to_f16_bits(b.y) << 16, f32_to_f16_bits(b.z) | f32_to_f16_bits(b.w) << 16);
+    }
+};
+template <> struct RowLoad<TCA_BF16, TCA_F32> {
+    __device__ static uint4 load(const void *B, long off)
+    {
+        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
+        const float4 a = p[0], b = p[1];
+        return make_uint4(f32_to_bf16_bits(a.x) | f32_to_bf16_bits(a.y) << 16, f32_to_bf16_bits(a.z) | f32_to_bf16_bits(a.w) << 16,
+                          f32_to_bf16_bits(b.x) | f32_to_bf16_bits(b.y) << 16, f32_to_bf16_bits(b.z) | f32_to_bf16_bits(b.w) << 16);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// LDS layouts (offsets in octets).  B tile: NT rows (n) x 32 k; two n-rows
+// share a 128-byte line of 8 octet slots, XOR-swizzled so that the 8 lanes of
+// an MFMA operand read (n = 16s+0..7, fixed q) and the 16 lanes of a
+// transposed write (8 n-segments x 2 k-halves) each hit distinct banks.
+// A tile: 16 rows x 32 k, same pairing with a 2-bit swizzle.
+// ---------------------------------------------------------------------------
+__device__ inline int b_oct(int n, int q)
+{
+    const int m = n >> 1;
+    const int sigma = ((m & 3) ^ ((m >> 2) & 7)) & 7;
+    return m * 8 + ((((n & 1) << 2) | q) ^ sigma);
+}
+__device__ inline int a_oct(int r, int q)
+{
+    const int m = r >> 1;
+    return m * 8 + ((((r & 1) << 2) | q) ^ (m & 3));
+}
+
+struct SpmmParams {
+    const int32_t *grp_chunk_ptr;
+    const int32_t *chunk_cols;
+    const int32_t *seg_ptr;
+    const uint16_t *entry_slot;
+    const int32_t *entry_src;
+    const int32_t *work_order;
+    const void *a;
+    const void *B;
+    float *C;
+    const float *row_scale;
+    const float *col_scale;
+    int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw;
+};
+
+template <int ES> struct Store;
+template <> struct Store<1> { using T = uint8_t; };
+template <> struct Store<2> { using T = uint16_t; };
+template <> struct Store<4> { using T = uint32_t; };
+template <> struct Store<8> { using T = uint2; };
+template <> struct Store<16> { using T = uint4; };
+
+// Pack element e of KPL octets (k-consecutive) into KPL*ES bytes.
+template <int KPL, int ES, class Oct>
+__device__ inline typename Store<KPL * ES>::T pack_k(const Oct (&rows)[KPL], int e)
+{
+    constexpr int W = (KPL * ES + 3) / 4;
+    uint32_t w[W > 0 ? W : 1] = {};
+#pragma unroll
+    for (int j = 0; j < KPL; j++) {
+        const uint32_t v = oct_elem(rows[j], e);
+        const int bit = j * ES * 8;
+        w[bit / 32] |= v << (bit % 32);
+    }
+    typename Store<KPL * ES>::T out;
+    if constexpr (KPL * ES == 1) out = (uint8_t)w[0];
+    else if constexpr (KPL * ES == 2) out = (uint16_t)w[0];
+    else if constexpr (KPL * ES == 4) out = w[0];
+    else if constexpr (KPL * ES == 8) out = make_uint2(w[0], w[1]);
+    else out = make_uint4(w[0], w[1], w[2], w[3]);
+    return out;
+}
+
+// One wave = one (group, N-tile).  RPW rows (PPG = RPW/16 panels) x NT columns.
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+__global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
+{
+    constexpr int PPG = RPW / 16;
+    constexpr int ES = Elt<AT>::ES;
+    constexpr int NSUB = NT / 16;
+    constexpr int SEGS = NT / 8;       // 8-column segments of a B row tile
+    constexpr int LPS = 64 / SEGS;     // lanes sharing one segment
+    constexpr int KPL = 32 / LPS;      // consecutive k rows per lane (= NT/16)
+    constexpr int OCT = 8 * ES;        // octet bytes
+    using Oct = typename Elt<AT>::Oct;
+    using E = typename Store<ES>::T;
+
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[NT * 32 * ES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_a[PPG * 512 * ES];
+
+    const int lane = threadIdx.x;
+    const int gi = blockIdx.x / p.n_tiles;
+    const int tile = blockIdx.x - gi * p.n_tiles;
+    const int g = p.work_order[gi];
+    const int n0 = tile * NT;
+    const int c_begin = p.grp_chunk_ptr[g], c_end = p.grp_chunk_ptr[g + 1];
+
+    f32x4 acc[PPG][NSUB];
+#pragma unroll
+    for (int i = 0; i < PPG; i++)
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) acc[i][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int seg = lane % SEGS, kq = lane / SEGS;
+    const int k0 = kq * KPL;
+    const int ncol = n0 + 8 * seg;
+    const bool col_ok = ncol < p.N;
+    const E *avals = static_cast<const E *>(p.a);
+
+    for (int c = c_begin; c < c_end; ++c) {
+        // 1. gather KPL condensed B rows (8 columns each) into registers
+        Oct rows[KPL];
+        const int32_t *cc = p.chunk_cols + (long)c * 32 + k0;
+#pragma unroll
+        for (int j = 0; j < KPL; j++) {
+            const int col = cc[j];
+            rows[j] = (col >= 0 && col_ok) ? RowLoad<AT, BT>::load(p.B, (long)col * p.ldb + ncol) : zero_oct<Oct>();
+        }
+        // 2. zero the A tiles, then scatter this chunk's entries
+#pragma unroll
+        for (int i = 0; i < PPG; i++)
+            reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PPG; i++) {
+            const int e0 = p.seg_ptr[c * PPG + i], e1 = p.seg_ptr[c * PPG + i + 1];
+            for (int e = e0 + lane; e < e1; e += 64) {
+                const int slot = p.entry_slot[e];
+                const E v = INDIRECT ? avals[p.entry_src[e]] : avals[e];
+                const int off = (i * 64 + a_oct(slot >> 5, (slot >> 3) & 3)) * OCT + (slot & 7) * ES;
+                *reinterpret_cast<E *>(lds_a + off) = v;
+            }
+        }
+        // 3. transposed write of the gathered rows: LDS[n][k0 .. k0+KPL)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int n = 8 * seg + e;
+            const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
+            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
+        }
+        __syncthreads();
+        // 4. operands and MFMAs
+        Oct a[PPG];
+#pragma unroll
+        for (int i = 0; i < PPG; i++)
+            a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            const Oct b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
+#pragma unroll
+            for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
+        }
+        __syncthreads();
+    }
+
+    // 5. epilogue: lane holds D[4*(lane/16)+j][lane%16] of each 16x16 tile
+    const int rbase = g * RPW + 4 * (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < PPG; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = rbase + 16 * i + j;
+            if (row >= p.n_rows) continue;
+            const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+            float *crow = p.C + (long)row * p.ldc;
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                const int n = n0 + 16 * s + (lane & 15);
+                if (n < p.N) {
+                    float v = acc[i][s][j] * rs;
+                    if (p.col_scale) v *= p.col_scale[n];
+                    crow[n] = v;
+                }
+            }
+        }
+    }
+}
+
+using SpmmFn = void (*)(SpmmParams);
+
+template <int RPW, tca_dtype AT, tca_dtype BT, bool IND>
+SpmmFn pick_nt(int nt)
+{
+    switch (nt) {
+    case 16: return spmm_kernel<RPW, 16, AT, BT, IND>;
+    case 32: return spmm_kernel<RPW, 32, AT, BT, IND>;
+    case 64: return spmm_kernel<RPW, 64, AT, BT, IND>;
+    case 128: return spmm_kernel<RPW, 128, AT, BT, IND>;
+    }
+    return nullptr;
+}
+
+template <tca_dtype AT, tca_dtype BT, bool IND>
+SpmmFn pick_rpw(int rpw, int nt)
+{
+    switch (rpw) {
+    case 16: return pick_nt<16, AT, BT, IND>(nt);
+    case 32: return pick_nt<32, AT, BT, IND>(nt);
+    case 64: return pick_nt<64, AT, BT, IND>(nt);
+    }
+    return nullptr;
+}
+
+template <bool IND>
+SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
+{
+    if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, IND>(rpw, nt);
+    if (at == TCA_BF16 && bt == TCA_BF16) return pick_rpw<TCA_BF16, TCA_BF16, IND>(rpw, nt);
+    if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_rpw<TCA_E4M3, TCA_E4M3, IND>(rpw, nt);
+    if (at == TCA_F16 && bt == TCA_F32) return pick_rpw<TCA_F16, TCA_F32, IND>(rpw, nt);
+    if (at == TCA_BF16 && bt == TCA_F32) return pick_rpw<TCA_BF16, TCA_F32, IND>(rpw, nt);
+    return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// SDDMM: out[e] = sum_n G[row(e), n] X[col(e), n] on the plan's pattern.
+// One wave = (group, block of CB chunks).  MFMA A operand = 8 consecutive n of
+// a G row, B operand = 8 consecutive n of a gathered X row: both are
+// contiguous in memory, so no LDS transpose is needed.  The 16x32 result tile
+// of each (chunk, panel) is staged in LDS and sampled at the entry slots.
+// ---------------------------------------------------------------------------
+struct SddmmParams {
+    const int32_t *grp_chunk_ptr;
+    const int32_t *chunk_cols;
+    const int32_t *seg_ptr;
+    const uint16_t *entry_slot;
+    const void *G;
+    const void *X;
+    float *out;
+    int n_rows, N, ldg, ldx, rpw, n_blocks_per_group_max;
+    const int32_t *unit_group;   // per workgroup: group id
+    const int32_t *unit_chunk;   // per workgroup: first chunk
+};
+
+template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
+__global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
+{
+    constexpr int PPG = RPW / 16;
+    using Oct = typename Elt<AT>::Oct;
+    __shared__ float tile[16 * 33];
+
+    const int lane = threadIdx.x;
+    const int g = p.unit_group[blockIdx.x];
+    const int c0 = p.unit_chunk[blockIdx.x];
+    const int c1 = min(c0 + CB, p.grp_chunk_ptr[g + 1]);
+    const int nc = c1 - c0;
+
+    f32x4 acc[CB][2][PPG];
+#pragma unroll
+    for (int b = 0; b < CB; b++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // X rows gathered by this lane: chunk b, half h -> column chunk_cols[(c0+b)*32 + 16h + lane%16]
+    int xcol[CB][2];
+#pragma unroll
+    for (int b = 0; b < CB; b++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) xcol[b][h] = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + 16 * h + (lane & 15)] : -1;
+    int grow[PPG];
+#pragma unroll
+    for (int i = 0; i < PPG; i++) grow[i] = g * RPW + 16 * i + (lane & 15);
+
+    const int q8 = 8 * (lane >> 4);
+    for (int n = 0; n < p.N; n += 32) {
+        Oct ga[PPG];
+#pragma unroll
+        for (int i = 0; i < PPG; i++)
+            ga[i] = grow[i] < p.n_rows ? RowLoad<AT, GT>::load(p.G, (long)grow[i] * p.ldg + n + q8) : zero_oct<Oct>();
+#pragma unroll
+        for (int b = 0; b < CB; b++) {
+            if (b >= nc) break;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int col = xcol[b][h];
+                const Oct xb = col >= 0 ? RowLoad<AT, AT>::load(p.X, (long)col * p.ldx + n + q8) : zero_oct<Oct>();
+#pragma unroll
+                for (int i = 0; i < PPG; i++) acc[b][h][i] = Elt<AT>::mma(ga[i], xb, acc[b][h][i]);
+            }
+        }
+    }
+    // sample: stage each (chunk, panel) 16x32 tile in LDS, then read entry slots
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+        if (b >= nc) break;
+#pragma unroll
+        for (int i = 0; i < PPG; i++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][i][j];
+            __syncthreads();
+            const int c = c0 + b;
+            const int e0 = p.seg_ptr[c * PPG + i], e1 = p.seg_ptr[c * PPG + i + 1];
+            for (int e = e0 + lane; e < e1; e += 64) {
+                const int slot = p.entry_slot[e];
+                p.out[e] = tile[(slot >> 5) * 33 + (slot & 31)];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small elementwise kernels
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void gather_kernel(const int32_t *__restrict__ idx, int n, const T *__restrict__ src, T *__restrict__ dst,
+                              bool scatter)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (scatter) dst[idx[i]] = src[i];
+        else dst[i] = src[idx[i]];
+    }
+}
+
+// f32 -> OCP E4M3FN, round to nearest even, saturating to +-448; the sign is
+// always kept (tiny negatives give 0x80).
+__device__ inline uint8_t f32_to_e4m3(float x)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7f;
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const float a = fabsf(x);
+    if (a >= 448.0f) return sign | 0x7e;
+    if (a < 0.015625f) {  // below 2^-6: multiples of 2^-9 (code 8 = 2^-6 on round-up)
+        const int m = (int)__builtin_rintf(a * 512.0f);
+        return (uint8_t)(sign | m);
+    }
+    const int e = (int)((u >> 23) & 0xff) - 127;
+    const uint32_t mant = u & 0x7fffffu;
+    const uint32_t keep = mant >> 20, rest = mant & 0xfffffu;
+    uint32_t code = ((uint32_t)(e + 7) << 3) | keep;
+    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) code++;
+    if (code > 0x7eu) code = 0x7eu;
+    return (uint8_t)(sign | code);
+}
+
+__device__ inline float f16_bits_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+__device__ inline float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// one wave per CSR row: scale = amax/448, q = e4m3(v / scale)
+__global__ __launch_bounds__(256) void quantize_rows_kernel(int n_rows, const int32_t *row_offset, const uint16_t *vals,
+                                                            uint8_t *q, float *scale)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n_rows) return;
+    const int e0 = row_offset[row], e1 = row_offset[row + 1];
+    float amax = 0.0f;
+    for (int e = e0 + lane; e < e1; e += 64) amax = fmaxf(amax, fabsf(f16_bits_to_f32(vals[e])));
+    amax = wave_max(amax);
+    const float s = amax > 0.0f ? amax / 448.0f : 1.0f;
+    if (lane == 0) scale[row] = s;
+    for (int e = e0 + lane; e < e1; e += 64) q[e] = f32_to_e4m3(f16_bits_to_f32(vals[e]) / s);
+}
+
+// 256 threads per 64 columns: 4 row phases reduce the column amax in LDS
+__global__ __launch_bounds__(256) void quantize_cols_kernel(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq,
+                                                            float *scale)
+{
+    __shared__ float part[4][64];
+    const int cl = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + cl;
+    float amax = 0.0f;
+    if (n < N)
+        for (int k = ph; k < K; k += 4) amax = fmaxf(amax, fabsf(f16_bits_to_f32(B[(long)k * ldb + n])));
+    part[ph][cl] = amax;
+    __syncthreads();
+    const float m = fmaxf(fmaxf(part[0][cl], part[1][cl]), fmaxf(part[2][cl], part[3][cl]));
+    const float s = m > 0.0f ? m / 448.0f : 1.0f;
+    if (n < N) {
+        if (ph == 0) scale[n] = s;
+        for (int k = ph; k < K; k += 4) q[(long)k * ldq + n] = f32_to_e4m3(f16_bits_to_f32(B[(long)k * ldb + n]) / s);
+    }
+}
+
+__global__ void warm_up_kernel() {}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+tca_status launch_warm_up(hipStream_t stream)
+{
+    hipLaunchKernelGGL(warm_up_kernel, dim3(1), dim3(64), 0, stream);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+int default_tile_n(const tca_plan *p, int N)
+{
+    (void)p;
+    if (N >= 128) return 64;
+    if (N >= 32) return 32;
+    return 16;
+}
+
+tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_dtype b_dtype, const void *B, int ldb,
+                       int N, float *C, int ldc, const float *row_scale, const float *col_scale, int tile_n,
+                       hipStream_t stream)
+{
+    const PlanTables &t = p->host;
+    if (tile_n == 0) tile_n = default_tile_n(p, N);
+    SpmmFn fn = p->transposed ? pick_spmm<true>(a_dtype, b_dtype, t.rpw, tile_n)
+                              : pick_spmm<false>(a_dtype, b_dtype, t.rpw, tile_n);
+    if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: unsupported dtype pair / rows_per_wave / tile_n");
+    SpmmParams prm;
+    prm.grp_chunk_ptr = p->d_grp_chunk_ptr;
+    prm.chunk_cols = p->d_chunk_cols;
+    prm.seg_ptr = p->d_seg_ptr;
+    prm.entry_slot = p->d_entry_slot;
+    prm.entry_src = p->d_entry_src;
+    prm.work_order = p->d_work_order;
+    prm.a = a;
+    prm.B = B;
+    prm.C = C;
+    prm.row_scale = row_scale;
+    prm.col_scale = col_scale;
+    prm.n_rows = t.n_rows;
+    prm.N = N;
+    prm.ldb = ldb;
+    prm.ldc = ldc;
+    prm.n_tiles = (N + tile_n - 1) / tile_n;
+    prm.n_groups = t.n_groups;
+    prm.rpw = t.rpw;
+    const long blocks = (long)t.n_groups * prm.n_tiles;
+    if (blocks <= 0) return TCA_OK;
+    if (blocks > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: grid too large");
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64), 0, stream, prm);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+namespace {
+using SddmmFn = void (*)(SddmmParams);
+constexpr int kSddmmCB = 4;
+
+template <tca_dtype AT, tca_dtype GT>
+SddmmFn pick_sddmm_rpw(int rpw)
+{
+    switch (rpw) {
+    case 16: return sddmm_kernel<16, kSddmmCB, AT, GT>;
+    case 32: return sddmm_kernel<32, kSddmmCB, AT, GT>;
+    case 64: return sddmm_kernel<64, kSddmmCB, AT, GT>;
+    }
+    return nullptr;
+}
+}  // namespace
+
+tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype, const void *X,
+                        int ldx, int N, float *out, hipStream_t stream)
+{
+    const PlanTables &t = p->host;
+    SddmmFn fn = nullptr;
+    if (x_dtype == TCA_F16 && (g_dtype == TCA_F16)) fn = pick_sddmm_rpw<TCA_F16, TCA_F16>(t.rpw);
+    else if (x_dtype == TCA_F16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_F16, TCA_F32>(t.rpw);
+    else if (x_dtype == TCA_BF16 && g_dtype == TCA_BF16) fn = pick_sddmm_rpw<TCA_BF16, TCA_BF16>(t.rpw);
+    else if (x_dtype == TCA_BF16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_BF16, TCA_F32>(t.rpw);
+    if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: unsupported dtype pair");
+    if (p->transposed) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: needs a forward plan");
+    // work units: (group, first chunk) in heavy-first group order
+    tca_plan *mp = const_cast<tca_plan *>(p);
+    if (!mp->d_sddmm_units) {
+        std::vector<int32_t> ug, uc;
+        for (int gi = 0; gi < t.n_groups; gi++) {
+            const int g = t.work_order[gi];
+            for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += kSddmmCB) {
+                ug.push_back(g);
+                uc.push_back(c);
+            }
+        }
+        mp->n_sddmm_units = (int)ug.size();
+        if (mp->n_sddmm_units > 0) {
+            TCA_HIP_TRY(hipMalloc(&mp->d_sddmm_units, sizeof(int32_t) * 2 * ug.size()));
+            int32_t *d = static_cast<int32_t *>(mp->d_sddmm_units);
+            TCA_HIP_TRY(hipMemcpy(d, ug.data(), sizeof(int32_t) * ug.size(), hipMemcpyHostToDevice));
+            TCA_HIP_TRY(hipMemcpy(d + ug.size(), uc.data(), sizeof(int32_t) * uc.size(), hipMemcpyHostToDevice));
+        }
+    }
+    if (mp->n_sddmm_units == 0) return TCA_OK;
+    SddmmParams prm;
+    prm.grp_chunk_ptr = p->d_grp_chunk_ptr;
+    prm.chunk_cols = p->d_chunk_cols;
+    prm.seg_ptr = p->d_seg_ptr;
+    prm.entry_slot = p->d_entry_slot;
+    prm.G = G;
+    prm.X = X;
+    prm.out = out;
+    prm.n_rows = t.n_rows;
+    prm.N = N;
+    prm.ldg = ldg;
+    prm.ldx = ldx;
+    prm.rpw = t.rpw;
+    prm.n_blocks_per_group_max = 0;
+    prm.unit_group = static_cast<const int32_t *>(mp->d_sddmm_units);
+    prm.unit_chunk = static_cast<const int32_t *>(mp->d_sddmm_units) + mp->n_sddmm_units;
+    hipLaunchKernelGGL(fn, dim3(mp->n_sddmm_units), dim3(64), 0, stream, prm);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *src, void *dst, bool scatter,
+                         hipStream_t stream)
+{
+    if (n <= 0) return TCA_OK;
+    const int threads = 256;
+    const int blocks = std::min((n + threads - 1) / threads, 65536);
+    switch (elem_bytes) {
+    case 1:
+        hipLaunchKernelGGL(gather_kernel<uint8_t>, dim3(blocks), dim3(threads), 0, stream, idx, n,
+                           static_cast<const uint8_t *>(src), static_cast<uint8_t *>(dst), scatter);
+        break;
+    case 2:
+        hipLaunchKernelGGL(gather_kernel<uint16_t>, dim3(blocks), dim3(threads), 0, stream, idx, n,
+                           static_cast<const uint16_t *>(src), static_cast<uint16_t *>(dst), scatter);
+        break;
+    case 4:
+        hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks), dim3(threads), 0, stream, idx, n,
+                           static_cast<const uint32_t *>(src), static_cast<uint32_t *>(dst), scatter);
+        break;
+    default: return fail(TCA_ERR_INVALID_ARG, "gather: element size");
+    }
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+tca_status launch_quantize_rows(int n_rows, const int32_t *row_offset, const uint16_t *vals, uint8_t *q, float *scale,
+                                hipStream_t stream)
+{
+    if (n_rows <= 0) return TCA_OK;
+    hipLaunchKernelGGL(quantize_rows_kernel, dim3((n_rows + 3) / 4), dim3(256), 0, stream, n_rows, row_offset, vals, q,
+                       scale);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq, float *scale,
+                                hipStream_t stream)
+{
+    if (K <= 0 || N <= 0) return TCA_OK;
+    hipLaunchKernelGGL(quantize_cols_kernel, dim3((N + 63) / 64), dim3(256), 0, stream, K, N, B, ldb, q, ldq, scale);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+}  // namespace tca

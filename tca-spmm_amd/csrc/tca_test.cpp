@@ -1,0 +1,67 @@
+// tca_test -- the drop-in counterpart of the reference driver (src/test.cu:29-88):
+// loads a .smtx pattern as a dense fp16 A (values 1.0), builds B (K x 256,
+// nonzero with probability 0.9, U[0,1) values -- here from a seeded stream),
+// computes the hipBLASLt gold, then checks rocSPARSE and our_spmm_balanced
+// against it with compare_mat and prints the three timings.
+#include "../../include/tca_spmm.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <vector>
+
+int main(int argc, char *argv[])
+{
+    if (argc <= 1) {
+        std::puts("Usage : ./tca_test {matrix.smtx} [N=256] [seed=0]");
+        std::exit(EXIT_FAILURE);
+    }
+    std::ifstream fin(argv[1], std::ifstream::in);
+    if (!fin.is_open()) {
+        std::fprintf(stderr, "cannot open %s\n", argv[1]);
+        std::exit(EXIT_FAILURE);
+    }
+    half *A;
+    int M, K, nnz;
+    smtx_input_to_Dense(fin, &A, argv[1], &M, &K, &nnz);
+    const int N = argc > 2 ? std::atoi(argv[2]) : 256;
+    const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 0;
+    double e_gemm = -1.0, e_sparse = -1.0, e_our = -1.0;
+    for (int i = 0; i < 10; i++) warm_up_gpu();
+    std::printf("Input file - %s\n", argv[1]);
+
+    // B: nonzero with probability 0.9, value U[0,1)  (src/test.cu:56, general.cuh:140-174)
+    std::vector<float> keep((size_t)K * N), val((size_t)K * N);
+    tca_gen_uniform_f32(seed, 11, keep.size(), 0.0, 1.0, keep.data());
+    tca_gen_uniform_f32(seed, 12, val.size(), 0.0, 1.0, val.data());
+    for (size_t i = 0; i < val.size(); i++)
+        if (keep[i] <= 0.1f) val[i] = 0.0f;
+    half *B = static_cast<half *>(std::malloc(sizeof(half) * (size_t)K * N));
+    tca_f32_to_f16_host(val.data(), val.size(), reinterpret_cast<uint16_t *>(B));
+    float *C_gold = static_cast<float *>(std::calloc((size_t)M * N, sizeof(float)));
+    float *C_cmp = static_cast<float *>(std::calloc((size_t)M * N, sizeof(float)));
+
+    perform_hipBLASLt_GemmEx(A, B, C_gold, M, N, K, 10, &e_gemm);
+
+    std::memset(C_cmp, 0, sizeof(float) * (size_t)M * N);
+    perform_rocSPARSE_SpMM(A, B, C_cmp, M, N, K, 10, &e_sparse);
+    if (!compare_mat(C_gold, C_cmp, M * N, DEFAULT_TOL)) std::printf("\033[35mvalidation failed (rocSPARSE)\033[0m\n");
+
+    CSR *csrA;
+    convert_dense_to_csr(A, &csrA, M, K);
+    std::memset(C_cmp, 0, sizeof(float) * (size_t)M * N);
+    our_spmm_balanced(csrA, B, C_cmp, M, N, K, 10, &e_our);
+    const bool ok = compare_mat(C_gold, C_cmp, M * N, DEFAULT_TOL);
+    if (!ok) std::printf("\033[35mvalidation failed (Our, balanced)\033[0m\n");
+
+    std::free(A);
+    destroy_csr(csrA);
+    std::free(B);
+    std::free(C_gold);
+    std::free(C_cmp);
+    std::fprintf(stdout, "GemmEx : \033[36m[%lf] ms\033[0m\n", e_gemm);
+    std::fprintf(stdout, "rocSPARSE: \033[36m[%lf] ms\033[0m \n", e_sparse);
+    std::fprintf(stdout, "our: \033[36m[%lf] ms\033[0m \n", e_our);
+    return ok ? 0 : 1;
+}

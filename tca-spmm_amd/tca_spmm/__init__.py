@@ -1,0 +1,19 @@
+"""tca_spmm -- MI355X (gfx950) row-balanced CSR x dense SpMM.
+
+Python mirror of the reference's host operator (mlsys-lab-sogang/TCA-SpMM,
+src/our_spmm.cu our_spmm_balanced) over the C ABI of libtca_spmm.so.  Every
+computation runs in the HIP library; a missing library raises ImportError.
+"""
+from ._lib import BF16, E4M3, F16, F32, TcaError, lib  # noqa: F401
+from . import gen, workloads  # noqa: F401
+
+lib()  # fail loudly at import when the native library is absent
+
+try:  # torch-facing API (torch is the device-memory layer, not the compute)
+    from .core import (Plan, hipblaslt_gemm, quantize_cols_e4m3, quantize_rows_e4m3,  # noqa: F401
+                       rocsparse_spmm, sddmm, spmm)
+except ImportError:  # pragma: no cover - torch absent
+    pass
+
+__all__ = ["Plan", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
+           "hipblaslt_gemm", "gen", "workloads", "TcaError", "F16", "BF16", "E4M3", "F32"]

@@ -1,0 +1,195 @@
+"""Host-side mirror of the reference operator for this path, over the C ABI.
+
+`Plan` is the MI355X counterpart of the reference's `workload_manager` tables
+(src/our_spmm.cu:347-471): a panel tiling of a CSR pattern.  `spmm` launches the
+HIP kernel that replaces `kernel_our_spmm_balanced` (src/our_spmm.cu:290-346).
+Torch tensors are used only as device buffers; every computation happens in
+libtca_spmm.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BF16, E4M3, F16, F32, call
+
+_TORCH_DTYPE = {torch.float16: F16, torch.bfloat16: BF16, torch.uint8: E4M3, torch.float32: F32}
+if hasattr(torch, "float8_e4m3fn"):
+    _TORCH_DTYPE[torch.float8_e4m3fn] = E4M3
+
+
+def tca_dtype(t):
+    try:
+        return _TORCH_DTYPE[t.dtype]
+    except KeyError:
+        raise TypeError(f"unsupported dtype {t.dtype}") from None
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _np_i32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _stream(stream=None):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Plan:
+    """Panel tiling of a CSR pattern (host tables + device copies).
+
+    rows_per_wave in {16, 32, 64}: consecutive rows sharing one condensed
+    column list (16-row MFMA panels, 1/2/4 per wave).  device=False builds a
+    host-only plan (inspection/export only; launches reject it).
+    """
+
+    def __init__(self, n_rows, n_cols, row_offset, col_idx, rows_per_wave=16, device=True, _handle=None):
+        self._h = None
+        if _handle is not None:
+            self._h = _handle
+        else:
+            rp = np.ascontiguousarray(row_offset, dtype=np.int32)
+            ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+            if rp.size != n_rows + 1:
+                raise ValueError("row_offset must have n_rows+1 entries")
+            if ci.size == 0:
+                ci = np.zeros(1, np.int32)
+            h = ctypes.c_void_p()
+            fn = "tca_plan_create" if device else "tca_plan_create_host"
+            call(fn, n_rows, n_cols, int(rp[-1]), _np_i32p(rp), _np_i32p(ci), rows_per_wave, ctypes.byref(h))
+            self._h = h
+        info = _lib.PlanInfo()
+        call("tca_plan_get_info", self._h, ctypes.byref(info))
+        self.n_rows, self.n_cols, self.nnz = info.n_rows, info.n_cols, info.nnz
+        self.rows_per_wave = info.rows_per_wave
+        self.n_groups, self.n_chunks = info.n_groups, info.n_chunks
+        self.max_group_chunks = info.max_group_chunks
+        self.transposed = bool(info.transposed)
+        self.device_bytes = info.device_bytes
+        self.build_ms = info.build_ms
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        return dict(n_rows=self.n_rows, n_cols=self.n_cols, nnz=self.nnz, rows_per_wave=self.rows_per_wave,
+                    n_groups=self.n_groups, n_chunks=self.n_chunks, max_group_chunks=self.max_group_chunks,
+                    transposed=self.transposed, device_bytes=self.device_bytes, build_ms=self.build_ms)
+
+    def export(self):
+        """Host copies of the tiling tables (numpy)."""
+        ppg = self.rows_per_wave // 16
+        gcp = np.zeros(self.n_groups + 1, np.int32)
+        cols = np.zeros(max(self.n_chunks * 32, 1), np.int32)
+        seg = np.zeros(self.n_chunks * ppg + 1, np.int32)
+        slot = np.zeros(max(self.nnz, 1), np.uint16)
+        src = np.zeros(max(self.nnz, 1), np.int32)
+        order = np.zeros(max(self.n_groups, 1), np.int32)
+        call("tca_plan_export", self._h, _np_i32p(gcp), _np_i32p(cols), _np_i32p(seg),
+             slot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _np_i32p(src), _np_i32p(order))
+        return dict(n_groups=self.n_groups, n_chunks=self.n_chunks, grp_chunk_ptr=gcp,
+                    chunk_cols=cols[:self.n_chunks * 32], seg_ptr=seg, entry_slot=slot[:self.nnz],
+                    entry_src=src[:self.nnz], work_order=order[:self.n_groups])
+
+    def transpose(self, rows_per_wave=None):
+        """Plan of A^T whose tiled values are read from this plan's tiled values."""
+        h = ctypes.c_void_p()
+        call("tca_plan_create_transpose", self._h, rows_per_wave or self.rows_per_wave, ctypes.byref(h))
+        return Plan(self.n_cols, self.n_rows, None, None, _handle=h)
+
+    def pack(self, values_csr, out=None, stream=None):
+        """Tiled-order copy of CSR-ordered device values (any supported dtype)."""
+        if out is None:
+            out = torch.empty_like(values_csr)
+        if self.nnz:
+            call("tca_pack_values", self._h, tca_dtype(values_csr), _ptr(values_csr), _ptr(out), _stream(stream))
+        return out
+
+    def unpack(self, values_tiled, out=None, stream=None):
+        """CSR-ordered copy of tiled-order device values."""
+        if out is None:
+            out = torch.empty_like(values_tiled)
+        if self.nnz:
+            call("tca_unpack_values", self._h, tca_dtype(values_tiled), _ptr(values_tiled), _ptr(out), _stream(stream))
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.lib().tca_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, stream=None):
+    """C[M x N] (float32) = A (plan + tiled values) @ B.
+
+    a_tiled: tiled values, float16 / bfloat16 / uint8 (E4M3 codes).  For a
+    transposed plan pass the forward plan's tiled values.  B: K x N row-major,
+    same dtype as A or float32 (converted in-kernel).  row_scale / col_scale
+    (float32, may be None) scale the FP32 result per row / column.
+    """
+    K, N = B.shape
+    if K != plan.n_cols:
+        raise ValueError(f"B has {K} rows, plan has {plan.n_cols} columns")
+    if out is None:
+        out = torch.empty((plan.n_rows, N), dtype=torch.float32, device=B.device)
+    at = tca_dtype(a_tiled) if a_tiled is not None else F16
+    call("tca_spmm", plan.handle, at, _ptr(a_tiled), tca_dtype(B), _ptr(B), B.stride(0), N, _ptr(out),
+         out.stride(0), _ptr(row_scale), _ptr(col_scale), tile_n, _stream(stream))
+    return out
+
+
+def sddmm(plan, G, X, out=None, stream=None):
+    """out_tiled[e] = sum_n G[row(e), n] X[col(e), n] (float32, tiled order)."""
+    M, N = G.shape
+    if M != plan.n_rows or X.shape[0] != plan.n_cols or X.shape[1] != N:
+        raise ValueError("sddmm: G must be M x N and X K x N")
+    if out is None:
+        out = torch.empty(max(plan.nnz, 1), dtype=torch.float32, device=G.device)[:plan.nnz]
+    call("tca_sddmm", plan.handle, tca_dtype(G), _ptr(G), G.stride(0), tca_dtype(X), _ptr(X), X.stride(0), N,
+         _ptr(out), _stream(stream))
+    return out
+
+
+def quantize_rows_e4m3(row_offset, values, stream=None):
+    """Per-row E4M3 codes (uint8) and FP32 scales of CSR values (float16, device)."""
+    n_rows = row_offset.numel() - 1
+    q = torch.empty(values.shape, dtype=torch.uint8, device=values.device)
+    scale = torch.empty(n_rows, dtype=torch.float32, device=values.device)
+    call("tca_quantize_rows_e4m3", n_rows, _ptr(row_offset), _ptr(values), _ptr(q), _ptr(scale), _stream(stream))
+    return q, scale
+
+
+def quantize_cols_e4m3(B, stream=None):
+    """Per-column E4M3 codes (uint8) and FP32 scales of a dense K x N float16 matrix."""
+    K, N = B.shape
+    q = torch.empty((K, N), dtype=torch.uint8, device=B.device)
+    scale = torch.empty(N, dtype=torch.float32, device=B.device)
+    call("tca_quantize_cols_e4m3", K, N, _ptr(B), B.stride(0), _ptr(q), N, _ptr(scale), _stream(stream))
+    return q, scale
+
+
+def rocsparse_spmm(M, N, K, row_offset, col_idx, values, B, out, n_iter=0):
+    """rocSPARSE CSR SpMM baseline (f16 -> f32); returns mean ms per call (n_iter > 0)."""
+    ms = ctypes.c_double(0.0)
+    call("tca_rocsparse_spmm", M, N, K, col_idx.numel(), _ptr(row_offset), _ptr(col_idx), _ptr(values), _ptr(B),
+         _ptr(out), n_iter, ctypes.byref(ms))
+    return ms.value
+
+
+def hipblaslt_gemm(M, N, K, A_dense, B, out, n_iter=0):
+    """hipBLASLt dense GEMM baseline (f16 -> f32); returns mean ms per call (n_iter > 0)."""
+    ms = ctypes.c_double(0.0)
+    call("tca_hipblaslt_gemm", M, N, K, _ptr(A_dense), _ptr(B), _ptr(out), n_iter, ctypes.byref(ms))
+    return ms.value
