@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Benchmark of the row-balanced CSR x dense FP16 SpMM on one MI355X.
+
+Workload (one "step"): the FFN-shaped GPU-forward suite -- CSR A of 2048x512
+and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 (lognormal(0.5) row lengths,
+values N(0,0.02)), B = K x 256 N(0,1), FP16 inputs, FP32 accumulate and output:
+ten SpMMs per step, inputs resident in HBM, the ten launches captured once in a
+HIP graph and replayed.  value = sum(2*nnz*N) / step time (TFLOP/s).
+
+Also reported: the HBM roofline of the SpMM kernel (algorithmic bytes per
+launch / measured launch time vs 8 TB/s), rocSPARSE and hipBLASLt on the same
+inputs, and the CPU oracle (FP32-accumulate C port of naive_gemm restricted to
+nonzeros) timed on the host cores.
+Prints one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "tca-spmm_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMM eff. TFLOP/s (2·nnz·N) + %HBM roofline; fwd+bwd ms vs plain FP16"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    read from the committed summary profiles/<name>.txt ('traffic_bytes_per_launch: X')."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-libs", action="store_true", help="skip rocSPARSE / hipBLASLt")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md and is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import tca_spmm as T
+    from tca_spmm import workloads
+    from tca_spmm.workloads import alg_bytes_fwd
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = 256
+    suite = workloads.plain_gpu_suite(seed=0, N=N)
+
+    def f16(bits):
+        return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
+
+    entries = []
+    for case in suite:
+        A = case.A
+        rpw, tile_n = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N)
+        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
+        tiled = plan.pack(f16(A.values))
+        B = f16(case.B)
+        C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
+        entries.append(dict(case=case, plan=plan, tiled=tiled, B=B, C=C, rpw=rpw, tile_n=tile_n,
+                            flops=2 * A.nnz * N, bytes=alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N)))
+    total_flops = sum(e["flops"] for e in entries)
+    total_bytes = sum(e["bytes"] for e in entries)
+
+    def step():
+        for e in entries:
+            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+
+    # parity gate before timing: every product against the CPU oracle
+    from oracle import oracle as O
+    step()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for e in entries:
+        A = e["case"].A
+        ref = O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B)
+        got = e["C"].cpu().numpy()
+        rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        worst = max(worst, rel)
+        assert rel <= 1e-5, f"{A.name}: rel-L2 {rel} vs oracle"
+    log(f"parity ok: worst rel-L2 vs CPU oracle {worst:.2e}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(max(3, args.warmup // 4)):
+        graph.replay()
+    torch.cuda.synchronize()
+
+    # timed region: K graph replays, an event pair around each (same stream)
+    K = args.steps
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    t_all0, t_all1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    if hasattr(torch.cuda, "_sleep"):
+        torch.cuda._sleep(20_000_000)        # queue backlog so host enqueue cost stays off the GPU clock
+    wall0 = time.perf_counter()
+    t_all0.record(stream)
+    for i in range(K):
+        starts[i].record(stream)
+        graph.replay()
+        ends[i].record(stream)
+    t_all1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = t_all0.elapsed_time(t_all1)
+    replay_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms_per_step = float(np.mean(replay_ms))
+    ms_per_step_total = ms_total / K
+    launch_us = ms_per_step * 1e3 / len(entries)
+    value = total_flops / (ms_per_step * 1e-3) / 1e12
+    achieved = total_bytes / (ms_per_step * 1e-3) / 1e9
+
+    # eager (no graph) for reference
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eager_ms = e0.elapsed_time(e1) / K
+
+    extra = {"graph_ms_per_step_incl_gaps": ms_per_step_total, "eager_ms_per_step": eager_ms,
+             "wall_s_timed": wall, "per_launch_us_mean": launch_us,
+             "configs": [{"name": e["case"].A.name, "nnz": e["case"].A.nnz, "rows_per_wave": e["rpw"],
+                          "tile_n": e["tile_n"]} for e in entries]}
+
+    # per-matrix times (eager events around each launch after a backlog)
+    per = []
+    for e in entries:
+        s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(3):
+            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+        if hasattr(torch.cuda, "_sleep"):
+            torch.cuda._sleep(2_000_000)
+        s.record(stream)
+        for _ in range(50):
+            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+        t.record(stream)
+        torch.cuda.synchronize()
+        us = s.elapsed_time(t) * 1e3 / 50
+        per.append({"name": e["case"].A.name, "us": us, "tflops": e["flops"] / (us * 1e-6) / 1e12,
+                    "hbm_frac": e["bytes"] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS})
+    extra["per_matrix"] = per
+
+    # library baselines on the same inputs
+    if not args.no_libs:
+        lib_ms = {"rocsparse": 0.0, "hipblaslt": 0.0}
+        for e in entries:
+            A = e["case"].A
+            rp = torch.from_numpy(A.row_offset).to(dev)
+            ci = torch.from_numpy(A.col_idx).to(dev)
+            vv = f16(A.values)
+            Cl = torch.empty_like(e["C"])
+            try:
+                lib_ms["rocsparse"] += T.rocsparse_spmm(A.n_rows, N, A.n_cols, rp, ci, vv, e["B"], Cl, n_iter=20)
+            except Exception as ex:  # report, do not hide
+                lib_ms["rocsparse"] = float("nan")
+                log("rocSPARSE failed:", ex)
+            dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
+            rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
+            dense[rows, ci.long()] = vv
+            try:
+                lib_ms["hipblaslt"] += T.hipblaslt_gemm(A.n_rows, N, A.n_cols, dense, e["B"], Cl, n_iter=20)
+            except Exception as ex:
+                lib_ms["hipblaslt"] = float("nan")
+                log("hipBLASLt failed:", ex)
+        extra["rocsparse_ms_per_step"] = lib_ms["rocsparse"]
+        extra["hipblaslt_ms_per_step"] = lib_ms["hipblaslt"]
+        extra["speedup_vs_rocsparse"] = lib_ms["rocsparse"] / ms_per_step if lib_ms["rocsparse"] else None
+        extra["speedup_vs_hipblaslt"] = lib_ms["hipblaslt"] / ms_per_step if lib_ms["hipblaslt"] else None
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cores = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            cores = min(16, os.cpu_count() or 1)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            for e in entries:
+                A = e["case"].A
+                O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B, cores)
+            reps += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": total_flops * reps / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+               "sample": f"the same ten-matrix suite (N=256) repeated {reps}x in {dt:.1f}s "
+                         f"({dt / reps * 1e3:.2f} ms per suite), FP32 accumulate, OpenMP over rows"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16",
+        "data": "synthetic",
+        "config": {"workload": "plain_gpu: CSR 2048x512 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 "
+                               "(lognormal(0.5) rows), B Kx256, FP16 in, FP32 accumulate/out; 10 SpMMs per step, "
+                               "one HIP graph replay",
+                   "N": N, "matrices": len(entries), "nnz_total": int(sum(e["case"].A.nnz for e in entries)),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("spmm_plain_gpu_suite_traffic"),
+                     "kernel": "spmm_kernel (10 launches per step; algorithmic bytes summed over the step / "
+                               "graph replay time)",
+                     "alg_bytes_per_step": total_bytes, "launch_us_mean": launch_us},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -251,19 +251,30 @@ int orc_compare_mat(const float *gold, const float *cmp, int n, float tol, int *
 void orc_spmm_f32(int M, int N, const int *rowptr, const int *colidx, const uint16_t *vals,
                   const uint16_t *B, int ldb, float *C, int ldc, int nthreads)
 {
+    /* B rows are decoded once (exact), then each output row accumulates
+     * a * B[col] in ascending column order in FP32. */
+    int K = 0;
+    for (int e = 0; e < rowptr[M]; e++) if (colidx[e] + 1 > K) K = colidx[e] + 1;
+    float *Bf = (float *)malloc(sizeof(float) * ((size_t)K * N + 1));
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (int k = 0; k < K; k++)
+        for (int n = 0; n < N; n++) Bf[(size_t)k * N + n] = orc_f16_to_f32(B[(size_t)k * ldb + n]);
+#ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
 #endif
     for (int r = 0; r < M; r++) {
         float *c = C + (size_t)r * ldc;
         for (int n = 0; n < N; n++) c[n] = 0.0f;
         for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
-            float a = orc_f16_to_f32(vals[e]);
-            const uint16_t *b = B + (size_t)colidx[e] * ldb;
-            for (int n = 0; n < N; n++) c[n] += a * orc_f16_to_f32(b[n]);
+            const float a = orc_f16_to_f32(vals[e]);
+            const float *b = Bf + (size_t)colidx[e] * N;
+            for (int n = 0; n < N; n++) c[n] += a * b[n];
         }
     }
+    free(Bf);
 }
 
 void orc_spmm_f64(int M, int N, const int *rowptr, const int *colidx, const double *vals,

@@ -5,7 +5,7 @@ src/our_spmm.cu our_spmm_balanced) over the C ABI of libtca_spmm.so.  Every
 computation runs in the HIP library; a missing library raises ImportError.
 """
 from ._lib import BF16, E4M3, F16, F32, TcaError, lib  # noqa: F401
-from . import gen, workloads  # noqa: F401
+from . import autotune, gen, workloads  # noqa: F401
 
 lib()  # fail loudly at import when the native library is absent
 

@@ -1,0 +1,105 @@
+"""Autotuner for the two launch parameters of the panel SpMM:
+rows_per_wave (16/32/64 rows sharing one condensed column list) and
+tile_n (16/32/64/128 output columns per wave).
+
+`tune()` times every candidate on the device with HIP events (launches queued
+behind a sleep so host enqueue cost stays off the clock) and returns them
+fastest first.  Results recorded with `record()` go to tuning_table.json next
+to this file; `choice_for()` looks up the nearest recorded shape (same M, K,
+N class, closest density) and falls back to a density rule when none exists.
+"""
+import json
+import math
+import os
+
+RPW_CANDIDATES = (16, 32, 64)
+TILE_CANDIDATES = (16, 32, 64, 128)
+TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_table.json")
+
+_table = None
+
+
+def _load():
+    global _table
+    if _table is None:
+        try:
+            with open(TABLE_PATH) as f:
+                _table = json.load(f)
+        except (OSError, ValueError):
+            _table = {"entries": []}
+    return _table
+
+
+def heuristic(M, K, nnz, N):
+    d = nnz / float(max(1, M * K))
+    rpw = 32 if d >= 0.05 else 16
+    tile = 64 if N >= 128 else (32 if N >= 32 else 16)
+    return rpw, tile
+
+
+def choice_for(M, K, nnz, N, op="spmm"):
+    """(rows_per_wave, tile_n) for an M x K matrix with nnz nonzeros and N columns."""
+    d = nnz / float(max(1, M * K))
+    best, best_dist = None, None
+    for e in _load()["entries"]:
+        if e.get("op", "spmm") != op or e["M"] != M or e["K"] != K:
+            continue
+        dist = abs(math.log(max(e["density"], 1e-6)) - math.log(max(d, 1e-6))) + abs(math.log(e["N"] / N))
+        if best is None or dist < best_dist:
+            best, best_dist = e, dist
+    if best is not None and best_dist < 0.7:
+        return best["rows_per_wave"], best["tile_n"]
+    return heuristic(M, K, nnz, N)
+
+
+def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda"):
+    """Time each (rows_per_wave, tile_n) on the device.  A: workloads.SparseMatrix
+    (fp16 values), B_dev: K x N float16 device tensor.  Returns a list of
+    dicts sorted by time (us per launch)."""
+    import numpy as np
+    import torch
+
+    from .core import Plan, spmm
+
+    if values_dev is None:
+        values_dev = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(device)
+    N = B_dev.shape[1]
+    cands = candidates or [(r, t) for r in RPW_CANDIDATES for t in TILE_CANDIDATES]
+    out = torch.empty((A.n_rows, N), dtype=torch.float32, device=device)
+    stream = torch.cuda.current_stream()
+    res = []
+    plans = {}
+    for rpw, tile in cands:
+        if rpw not in plans:
+            p = Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
+            plans[rpw] = (p, p.pack(values_dev))
+        p, tiled = plans[rpw]
+        for _ in range(3):
+            spmm(p, tiled, B_dev, out=out, tile_n=tile)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if hasattr(torch.cuda, "_sleep"):
+            torch.cuda._sleep(1_000_000)
+        s.record(stream)
+        for _ in range(iters):
+            spmm(p, tiled, B_dev, out=out, tile_n=tile)
+        e.record(stream)
+        torch.cuda.synchronize()
+        res.append({"rows_per_wave": rpw, "tile_n": tile, "us": s.elapsed_time(e) * 1e3 / iters})
+    res.sort(key=lambda r: r["us"])
+    return res
+
+
+def record(A, N, results, op="spmm", path=None):
+    """Store the fastest candidate for this shape in the tuning table."""
+    tab = _load()
+    d = A.nnz / float(A.n_rows * A.n_cols)
+    best = results[0]
+    tab["entries"] = [e for e in tab["entries"]
+                      if not (e.get("op", "spmm") == op and e["M"] == A.n_rows and e["K"] == A.n_cols
+                              and e["N"] == N and abs(e["density"] - d) < 1e-3)]
+    tab["entries"].append({"op": op, "M": A.n_rows, "K": A.n_cols, "N": N, "density": round(d, 5),
+                           "rows_per_wave": best["rows_per_wave"], "tile_n": best["tile_n"], "us": best["us"],
+                           "candidates": results})
+    with open(path or TABLE_PATH, "w") as f:
+        json.dump(tab, f, indent=1)
