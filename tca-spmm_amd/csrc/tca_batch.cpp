@@ -1,34 +1,147 @@
-// Batched persistent launch (placeholder until the persistent kernel lands).
+// Batched persistent launch: the host builds one descriptor per item and one
+// record per work unit (heaviest first), uploads both once, and every run is a
+// single kernel launch that drains the unit list (see batch_kernel).
 #include "tca_internal.hpp"
+
+#include <algorithm>
+#include <numeric>
 
 using tca::fail;
 
 struct tca_batch {
-    int n_units = 0;
+    int n_items = 0, n_units = 0, grid = 0, device = 0;
+    void *d_mem = nullptr;  // items | units | counters
+    tca::BatchItemDev *d_items = nullptr;
+    tca::BatchUnitDev *d_units = nullptr;
+    int *d_counters = nullptr;
 };
+
+namespace {
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
 
 extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch)
 {
-    (void)n_items;
-    (void)items;
-    if (batch) *batch = nullptr;
-    return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: not implemented yet");
+    if (!batch || n_items <= 0 || !items) return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: bad argument");
+    *batch = nullptr;
+    std::vector<tca::BatchItemDev> dev(n_items);
+    std::vector<tca::BatchUnitDev> units;
+    std::vector<double> cost;
+    for (int i = 0; i < n_items; i++) {
+        const tca_batch_item &it = items[i];
+        const tca_plan *p = it.plan;
+        if (!p || !p->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: item without a device plan");
+        const tca::PlanTables &t = p->host;
+        const int ppg = t.rpw / 16;
+        tca::BatchItemDev &d = dev[i];
+        d = tca::BatchItemDev{};
+        d.n_rows = t.n_rows;
+        d.N = it.N;
+        d.rpw = t.rpw;
+        d.grp_chunk_ptr = p->d_grp_chunk_ptr;
+        d.chunk_cols = p->d_chunk_cols;
+        d.seg_ptr = p->d_seg_ptr;
+        d.entry_slot = p->d_entry_slot;
+        d.entry_src = p->d_entry_src;
+        d.a = it.a;
+        d.b = it.b;
+        d.out = it.out;
+        if (it.op == TCA_OP_SPMM) {
+            const int tile = it.tile_n ? it.tile_n : tca::default_tile_n(p, it.N);
+            if (it.N <= 0 || it.N % 8 || it.ldb < it.N || it.ldc < it.N || it.ldb % 8 || it.ldc % 4 || !it.b || !it.out ||
+                !aligned16(it.b) || !aligned16(it.out) || (t.nnz > 0 && !it.a))
+                return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SPMM item " + std::to_string(i) + " has bad shapes/buffers");
+            d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0);
+            if (d.variant < 0)
+                return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
+                                                     " (batched SpMM: f16 A with f16 B forward, f32 B transposed)");
+            d.ldb = it.ldb;
+            d.ldc = it.ldc;
+            d.n_tiles = (it.N + tile - 1) / tile;
+            for (int gi = 0; gi < t.n_groups; gi++) {
+                const int g = t.work_order[gi];
+                const int nch = t.grp_chunk_ptr[g + 1] - t.grp_chunk_ptr[g];
+                for (int tt = 0; tt < d.n_tiles; tt++) {
+                    units.push_back({i, g, tt, 0});
+                    cost.push_back(nch * (2.0 + ppg * tile / 16.0) + 4.0);   // ~ MFMAs + staging per chunk
+                }
+            }
+        } else if (it.op == TCA_OP_SDDMM) {
+            if (p->transposed) return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM needs a forward plan");
+            if (it.N <= 0 || it.N % 32 || it.lda < it.N || it.ldb < it.N || it.lda % 8 || it.ldb % 8 || !it.a || !it.b ||
+                !aligned16(it.a) || !aligned16(it.b) || (t.nnz > 0 && !it.out))
+                return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM item " + std::to_string(i) + " has bad shapes/buffers");
+            d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false);
+            if (d.variant < 0) return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i));
+            d.ldb = it.ldb;   // X
+            d.ldc = it.lda;   // G
+            for (int gi = 0; gi < t.n_groups; gi++) {
+                const int g = t.work_order[gi];
+                for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {
+                    const int nc = std::min(tca::kBatchSddmmCB, t.grp_chunk_ptr[g + 1] - c);
+                    units.push_back({i, g, c, 0});
+                    cost.push_back(nc * 2.0 * ppg * (it.N / 32.0) + 4.0);
+                }
+            }
+        } else {
+            return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: unknown op");
+        }
+    }
+    // heaviest units first
+    std::vector<int> order(units.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<tca::BatchUnitDev> sorted(units.size());
+    for (size_t k = 0; k < order.size(); k++) sorted[k] = units[order[k]];
+
+    tca_batch *b = new (std::nothrow) tca_batch();
+    if (!b) return fail(TCA_ERR_ALLOC, "tca_batch_create: host allocation");
+    b->n_items = n_items;
+    b->n_units = (int)sorted.size();
+    TCA_HIP_TRY(hipGetDevice(&b->device));
+    int cus = 0;
+    TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    b->grid = std::max(1, std::min(b->n_units, std::max(cus, 1) * 8));
+    const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
+    const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
+    const size_t total = bi + bu + 256;
+    if (hipMalloc(&b->d_mem, total) != hipSuccess) { delete b; return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation"); }
+    char *m = static_cast<char *>(b->d_mem);
+    b->d_items = reinterpret_cast<tca::BatchItemDev *>(m);
+    b->d_units = reinterpret_cast<tca::BatchUnitDev *>(m + bi);
+    b->d_counters = reinterpret_cast<int *>(m + bi + bu);
+    if (hipMemcpy(b->d_items, dev.data(), bi, hipMemcpyHostToDevice) != hipSuccess ||
+        (!sorted.empty() && hipMemcpy(b->d_units, sorted.data(), sizeof(tca::BatchUnitDev) * sorted.size(),
+                                      hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemset(b->d_counters, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        tca_batch_destroy(b);
+        return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
+    }
+    *batch = b;
+    return TCA_OK;
 }
+
 extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
 {
-    (void)batch;
-    (void)stream;
-    return fail(TCA_ERR_UNSUPPORTED, "tca_batch_run: not implemented yet");
+    if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
+    return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->grid,
+                             static_cast<hipStream_t>(stream));
 }
+
 extern "C" tca_status tca_batch_destroy(tca_batch *batch)
 {
+    if (!batch) return TCA_OK;
+    if (batch->d_mem) (void)hipFree(batch->d_mem);
     delete batch;
     return TCA_OK;
 }
+
 extern "C" tca_status tca_batch_get_info(const tca_batch *batch, int *n_units, int *n_workgroups)
 {
-    (void)batch;
-    if (n_units) *n_units = 0;
-    if (n_workgroups) *n_workgroups = 0;
-    return fail(TCA_ERR_UNSUPPORTED, "tca_batch_get_info: not implemented yet");
+    if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_get_info: NULL batch");
+    if (n_units) *n_units = batch->n_units;
+    if (n_workgroups) *n_workgroups = batch->grid;
+    return TCA_OK;
 }

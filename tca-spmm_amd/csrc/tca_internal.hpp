@@ -80,4 +80,23 @@ tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_
 int default_tile_n(const tca_plan *p, int N);
 tca_status launch_warm_up(hipStream_t stream);
 
+// Batched persistent launch: one descriptor per item, one record per work unit.
+struct BatchItemDev {
+    int32_t variant;   // kernel variant id (batch_variant)
+    int32_t n_rows, N, ldb, ldc, n_tiles, rpw, pad;
+    const int32_t *grp_chunk_ptr, *chunk_cols, *seg_ptr;
+    const uint16_t *entry_slot;
+    const int32_t *entry_src;
+    const void *a, *b;
+    float *out;
+};
+struct BatchUnitDev {
+    int32_t item, g, t, pad;   // SPMM: (group, N-tile); SDDMM: (group, first chunk)
+};
+constexpr int kBatchSddmmCB = 4;
+// -1 when the combination has no batched variant.
+int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed);
+tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
+                        hipStream_t stream);
+
 }  // namespace tca

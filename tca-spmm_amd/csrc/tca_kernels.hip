@@ -167,9 +167,10 @@ __device__ inline typename Store<KPL * ES>::T pack_k(const Oct (&rows)[KPL], int
     return out;
 }
 
-// One wave = one (group, N-tile).  RPW rows (PPG = RPW/16 panels) x NT columns.
+// One wave = one (group g, N-tile `tile`).  RPW rows (PPG = RPW/16 panels) x NT
+// columns.  lds_b holds NT*32 elements, lds_a PPG*512 elements.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
+__device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, uint8_t *lds_b, uint8_t *lds_a)
 {
     constexpr int PPG = RPW / 16;
     constexpr int ES = Elt<AT>::ES;
@@ -181,13 +182,7 @@ __global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
     using Oct = typename Elt<AT>::Oct;
     using E = typename Store<ES>::T;
 
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[NT * 32 * ES];
-    __shared__ __attribute__((aligned(16))) uint8_t lds_a[PPG * 512 * ES];
-
     const int lane = threadIdx.x;
-    const int gi = blockIdx.x / p.n_tiles;
-    const int tile = blockIdx.x - gi * p.n_tiles;
-    const int g = p.work_order[gi];
     const int n0 = tile * NT;
     const int c_begin = p.grp_chunk_ptr[g], c_end = p.grp_chunk_ptr[g + 1];
 
@@ -272,6 +267,16 @@ __global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
     }
 }
 
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+__global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[NT * 32 * Elt<AT>::ES];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_a[(RPW / 16) * 512 * Elt<AT>::ES];
+    const int gi = blockIdx.x / p.n_tiles;
+    const int tile = blockIdx.x - gi * p.n_tiles;
+    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, p.work_order[gi], tile, lds_b, lds_a);
+}
+
 using SpmmFn = void (*)(SpmmParams);
 
 template <int RPW, tca_dtype AT, tca_dtype BT, bool IND>
@@ -328,16 +333,14 @@ struct SddmmParams {
     const int32_t *unit_chunk;   // per workgroup: first chunk
 };
 
+// One wave = (group g, chunks [c0, c0+CB) of that group); `tile` is 16*33 floats of LDS.
 template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
+__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, float *tile)
 {
     constexpr int PPG = RPW / 16;
     using Oct = typename Elt<AT>::Oct;
-    __shared__ float tile[16 * 33];
 
     const int lane = threadIdx.x;
-    const int g = p.unit_group[blockIdx.x];
-    const int c0 = p.unit_chunk[blockIdx.x];
     const int c1 = min(c0 + CB, p.grp_chunk_ptr[g + 1]);
     const int nc = c1 - c0;
 
@@ -397,6 +400,13 @@ __global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
             __syncthreads();
         }
     }
+}
+
+template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
+__global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
+{
+    __shared__ float tile[16 * 33];
+    sddmm_unit<RPW, CB, AT, GT>(p, p.unit_group[blockIdx.x], p.unit_chunk[blockIdx.x], tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +491,137 @@ __global__ __launch_bounds__(256) void quantize_cols_kernel(int K, int N, const 
 
 __global__ void warm_up_kernel() {}
 
+// ---------------------------------------------------------------------------
+// Batched persistent kernel: every workgroup (one wave) takes work units from a
+// shared counter until the list is exhausted, so one launch covers the
+// forward SpMM, the transposed SpMM and the SDDMM of many layers.  Units are
+// sorted heaviest first on the host.  counters[0] hands out units,
+// counters[1] counts finished workgroups; the last one resets both, so the
+// next launch (stream-ordered) starts from zero without a memset.
+// Variant ids: 0..11 forward f16 x f16 (rpw index * 4 + tile index),
+// 12..23 transposed f16 x f32, 24..26 SDDMM G f32, 27..29 SDDMM G f16.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
+{
+    SpmmParams p;
+    p.grp_chunk_ptr = d.grp_chunk_ptr;
+    p.chunk_cols = d.chunk_cols;
+    p.seg_ptr = d.seg_ptr;
+    p.entry_slot = d.entry_slot;
+    p.entry_src = d.entry_src;
+    p.work_order = nullptr;
+    p.a = d.a;
+    p.B = d.b;
+    p.C = d.out;
+    p.row_scale = nullptr;
+    p.col_scale = nullptr;
+    p.n_rows = d.n_rows;
+    p.N = d.N;
+    p.ldb = d.ldb;
+    p.ldc = d.ldc;
+    p.n_tiles = d.n_tiles;
+    p.n_groups = 0;
+    p.rpw = d.rpw;
+    return p;
+}
+
+__device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
+{
+    SddmmParams p;
+    p.grp_chunk_ptr = d.grp_chunk_ptr;
+    p.chunk_cols = d.chunk_cols;
+    p.seg_ptr = d.seg_ptr;
+    p.entry_slot = d.entry_slot;
+    p.G = d.a;
+    p.X = d.b;
+    p.out = d.out;
+    p.n_rows = d.n_rows;
+    p.N = d.N;
+    p.ldg = d.ldc;
+    p.ldx = d.ldb;
+    p.rpw = d.rpw;
+    p.n_blocks_per_group_max = 0;
+    p.unit_group = nullptr;
+    p.unit_chunk = nullptr;
+    return p;
+}
+
+#define TCA_FWD_CASE(id, R, T) \
+    case id: spmm_unit<R, T, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, lds_b, lds_a); break;
+#define TCA_BWD_CASE(id, R, T) \
+    case id: spmm_unit<R, T, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, lds_b, lds_a); break;
+
+__global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restrict__ items,
+                                                   const BatchUnitDev *__restrict__ units, int n_units, int *counters)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[128 * 32 * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_a[4 * 512 * 2];
+    const int lane = threadIdx.x;
+    for (;;) {
+        int u = 0;
+        if (lane == 0) u = atomicAdd(&counters[0], 1);
+        u = __shfl(u, 0);
+        if (u >= n_units) break;
+        const BatchUnitDev un = units[u];
+        const BatchItemDev d = items[un.item];
+        switch (d.variant) {
+            TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64) TCA_FWD_CASE(3, 16, 128)
+            TCA_FWD_CASE(4, 32, 16) TCA_FWD_CASE(5, 32, 32) TCA_FWD_CASE(6, 32, 64) TCA_FWD_CASE(7, 32, 128)
+            TCA_FWD_CASE(8, 64, 16) TCA_FWD_CASE(9, 64, 32) TCA_FWD_CASE(10, 64, 64) TCA_FWD_CASE(11, 64, 128)
+            TCA_BWD_CASE(12, 16, 16) TCA_BWD_CASE(13, 16, 32) TCA_BWD_CASE(14, 16, 64) TCA_BWD_CASE(15, 16, 128)
+            TCA_BWD_CASE(16, 32, 16) TCA_BWD_CASE(17, 32, 32) TCA_BWD_CASE(18, 32, 64) TCA_BWD_CASE(19, 32, 128)
+            TCA_BWD_CASE(20, 64, 16) TCA_BWD_CASE(21, 64, 32) TCA_BWD_CASE(22, 64, 64) TCA_BWD_CASE(23, 64, 128)
+        case 24: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 25: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 26: sddmm_unit<64, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 27: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 28: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 29: sddmm_unit<64, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        default: break;
+        }
+        __syncthreads();  // LDS is reused by the next unit
+    }
+    if (lane == 0) {
+        __threadfence();
+        if (atomicAdd(&counters[1], 1) == (int)gridDim.x - 1) {
+            atomicExch(&counters[0], 0);
+            atomicExch(&counters[1], 0);
+        }
+    }
+}
+#undef TCA_FWD_CASE
+#undef TCA_BWD_CASE
+
+}  // namespace
+
+int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
+{
+    const int ri = rpw == 16 ? 0 : rpw == 32 ? 1 : rpw == 64 ? 2 : -1;
+    const int ti = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : tile_n == 128 ? 3 : -1;
+    if (ri < 0) return -1;
+    if (op == TCA_OP_SPMM) {
+        if (ti < 0 || a_dtype != TCA_F16) return -1;
+        if (!transposed && b_dtype == TCA_F16) return ri * 4 + ti;
+        if (transposed && b_dtype == TCA_F32) return 12 + ri * 4 + ti;
+        return -1;
+    }
+    if (op == TCA_OP_SDDMM && b_dtype == TCA_F16 && !transposed) {
+        if (a_dtype == TCA_F32) return 24 + ri;
+        if (a_dtype == TCA_F16) return 27 + ri;
+    }
+    return -1;
+}
+
+tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
+                        hipStream_t stream)
+{
+    if (n_units <= 0) return TCA_OK;
+    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(64), 0, stream, items, units, n_units, counters);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+namespace {
 }  // namespace
 
 // ---------------------------------------------------------------------------

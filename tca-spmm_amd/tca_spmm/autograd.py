@@ -1,0 +1,64 @@
+"""Sparse-weight training: Y = W X with a CSR weight W, differentiable in W and X.
+
+Backward (hand-written, both on the C ABI):
+  dX = W^T G        transposed-plan SpMM (G read as float32, rounded to the
+                    weight's MFMA type on load)
+  dW = SDDMM(G, X^T) sampled on W's pattern, FP32, in tiled order
+W's values live in *tiled order* (the plan's entry order) as the parameter,
+so no permutation runs per step; `SparseLinear.csr_values()` exports them.
+"""
+import numpy as np
+import torch
+
+from .core import Plan, sddmm, spmm
+
+
+class SpmmFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, w_tiled, X, plan, plan_t, tile_n):
+        Y = spmm(plan, w_tiled, X, tile_n=tile_n)
+        ctx.save_for_backward(w_tiled, X)
+        ctx.plan, ctx.plan_t, ctx.tile_n = plan, plan_t, tile_n
+        return Y
+
+    @staticmethod
+    def backward(ctx, G):
+        w_tiled, X = ctx.saved_tensors
+        G = G.contiguous()
+        if G.dtype != torch.float32:
+            G = G.float()
+        dW = dX = None
+        if ctx.needs_input_grad[0]:
+            dW = sddmm(ctx.plan, G, X).to(w_tiled.dtype)
+        if ctx.needs_input_grad[1]:
+            dX = spmm(ctx.plan_t, w_tiled, G, tile_n=ctx.tile_n).to(X.dtype)
+        return dW, dX, None, None, None
+
+
+def sparse_matmul(w_tiled, X, plan, plan_t, tile_n=0):
+    """Y (float32) = W X, differentiable in the tiled weight values and X."""
+    return SpmmFunction.apply(w_tiled, X, plan, plan_t, tile_n)
+
+
+class SparseLinear(torch.nn.Module):
+    """A CSR-pattern weight (n_out x n_in) applied as Y = W X to X (n_in x N).
+
+    The pattern is fixed at construction; the values are an fp16/bf16
+    parameter in tiled order.
+    """
+
+    def __init__(self, n_out, n_in, row_offset, col_idx, values_csr, rows_per_wave=16, rows_per_wave_t=None,
+                 tile_n=0, device="cuda"):
+        super().__init__()
+        self.plan = Plan(n_out, n_in, row_offset, col_idx, rows_per_wave=rows_per_wave)
+        self.plan_t = self.plan.transpose(rows_per_wave_t or rows_per_wave)
+        v = values_csr if isinstance(values_csr, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(values_csr).view(np.int16)).view(torch.float16)
+        self.weight = torch.nn.Parameter(self.plan.pack(v.to(device)))
+        self.tile_n = tile_n
+
+    def forward(self, X):
+        return sparse_matmul(self.weight, X, self.plan, self.plan_t, self.tile_n)
+
+    def csr_values(self, tiled=None):
+        return self.plan.unpack((self.weight if tiled is None else tiled).detach())
