@@ -45,8 +45,145 @@ def load_traffic(name):
     return None
 
 
+def time_graph(torch, fn, iters):
+    """Mean ms of fn() captured in a HIP graph and replayed back to back
+    (event pair on the capture-free current stream, queue backlog first)."""
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    for _ in range(5):
+        g.replay()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if hasattr(torch.cuda, "_sleep"):
+        torch.cuda._sleep(5_000_000)
+    s.record(st)
+    for _ in range(iters):
+        g.replay()
+    e.record(st)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / iters
+
+
+def time_eager(torch, fn, iters):
+    """Mean ms of fn() launched eagerly (host launch cost included when the
+    GPU outruns the host)."""
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(st)
+    for _ in range(iters):
+        fn()
+    e.record(st)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / iters
+
+
+def bench_fwd_bwd(torch, np, T, O, dev):
+    """Twelve uneven layers, forward + dX + dW: per-call path (36 launches) vs
+    one persistent launch replayed from a HIP graph."""
+    from tca_spmm import workloads
+    from tca_spmm.workloads import alg_bytes_dw, alg_bytes_dx, alg_bytes_fwd
+    layers, items, flops, nbytes = [], [], 0, 0
+    for case in workloads.batch12(seed=0):
+        W, X, G = case.A, case.B, case.extra["G"]
+        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N)
+        plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+        plan_t = plan.transpose()
+        w = plan.pack(torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev))
+        Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
+        Gd = torch.from_numpy(G).to(dev)
+        L = dict(plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile,
+                 Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
+                 dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
+                 dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
+        layers.append(L)
+        items += [T.spmm_item(plan, w, Xd, L["Y"], tile), T.spmm_item(plan_t, w, Gd, L["dX"], tile),
+                  T.sddmm_item(plan, Gd, Xd, L["dW"])]
+        flops += 3 * 2 * W.nnz * case.N
+        nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
+                   + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, case.N))
+
+    def per_call():
+        for L in layers:
+            T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
+            T.spmm(L["plan_t"], L["w"], L["G"], out=L["dX"], tile_n=L["tile"])
+            T.sddmm(L["plan"], L["G"], L["X"], out=L["dW"])
+    bt = T.Batch(items)
+    plain_eager = time_eager(torch, per_call, 20)
+    plain_graph = time_graph(torch, per_call, 20)
+    fused = time_graph(torch, bt.run, 50)
+    return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph, "fused_ms": fused,
+            "speedup_vs_eager": plain_eager / fused, "speedup_vs_graph": plain_graph / fused,
+            "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "launches_plain": 3 * len(layers), "units": bt.n_units}
+
+
+def bench_precision(torch, np, T, O, dev):
+    """4096x4096 at 0.9, N=512: FP16, BF16 and FP8-E4M3 (row/column scales)
+    throughput, with rel-L2 / max-abs against the FP64 oracle on the original
+    values (format + kernel error) and on the quantised values (kernel error)."""
+    from tca_spmm import workloads
+    from tca_spmm.workloads import alg_bytes_fwd
+    case = workloads.precision(seed=0)
+    A, N = case.A, case.N
+    rp, ci = A.row_offset, A.col_idx
+    a16 = A.values.view(np.float16)
+    b16 = case.B.view(np.float16)
+    ref = O.spmm_f64(A.n_rows, N, rp, ci, a16.astype(np.float64), b16.astype(np.float64))
+    rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N)
+    plan = T.Plan(A.n_rows, A.n_cols, rp, ci, rpw)
+    out = {}
+
+    def err(C, r):
+        d = C.astype(np.float64) - r
+        return float(np.linalg.norm(d) / np.linalg.norm(r)), float(np.abs(d).max())
+
+    va = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(dev)
+    vb = torch.from_numpy(case.B.view(np.int16)).view(torch.float16).to(dev)
+    C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
+    for name, cast in (("f16", torch.float16), ("bf16", torch.bfloat16)):
+        ta, tb = plan.pack(va.to(cast)), vb.to(cast)
+        ms = time_graph(torch, lambda: T.spmm(plan, ta, tb, out=C, tile_n=tile), 50)
+        Cn = C.cpu().numpy()
+        rq = O.spmm_f64(A.n_rows, N, rp, ci, plan.unpack(ta).float().cpu().numpy().astype(np.float64),
+                        tb.float().cpu().numpy().astype(np.float64))
+        e_all, m_all = err(Cn, ref)
+        e_q, m_q = err(Cn, rq)
+        out[name] = {"us": ms * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+                     "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
+                     "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q}
+    rpd = torch.from_numpy(rp).to(dev)
+    qa, sa = T.quantize_rows_e4m3(rpd, va)
+    qb, sb = T.quantize_cols_e4m3(vb)
+    ta = plan.pack(qa)
+    ms = time_graph(torch, lambda: T.spmm(plan, ta, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile), 50)
+    Cn = C.cpu().numpy()
+    rq = O.spmm_f64(A.n_rows, N, rp, ci, O.e4m3_to_f32(qa.cpu().numpy()).astype(np.float64),
+                    O.e4m3_to_f32(qb.cpu().numpy()).astype(np.float64))
+    rq *= sa.cpu().numpy().astype(np.float64)[:, None] * sb.cpu().numpy().astype(np.float64)[None, :]
+    e_all, m_all = err(Cn, ref)
+    e_q, m_q = err(Cn, rq)
+    qms = time_graph(torch, lambda: T.quantize_cols_e4m3(vb), 20)
+    out["e4m3"] = {"us": ms * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+                   "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N, 1, 1, True) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
+                   "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q,
+                   "quantize_cols_us": qms * 1e3}
+    out["config"] = {"rows_per_wave": rpw, "tile_n": tile, "nnz": A.nnz, "N": N}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--sections", default="batched,fwdbwd,precision",
+                    help="comma list of extra sections: batched, fwdbwd, precision")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -54,6 +191,7 @@ def main():
     ap.add_argument("--no-libs", action="store_true", help="skip rocSPARSE / hipBLASLt")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     args = ap.parse_args()
+    args.sections = set(s for s in args.sections.split(",") if s)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described in DESIGN.md and is not run.", file=sys.stderr)
@@ -203,6 +341,26 @@ def main():
         extra["hipblaslt_ms_per_step"] = lib_ms["hipblaslt"]
         extra["speedup_vs_rocsparse"] = lib_ms["rocsparse"] / ms_per_step if lib_ms["rocsparse"] else None
         extra["speedup_vs_hipblaslt"] = lib_ms["hipblaslt"] / ms_per_step if lib_ms["hipblaslt"] else None
+
+    # the same ten products as ONE persistent launch
+    if "batched" in args.sections:
+        bt = T.Batch([T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], e["tile_n"]) for e in entries])
+        us = time_graph(torch, lambda: bt.run(), 100) * 1e3
+        extra["batched_suite"] = {"us_per_launch": us, "tflops": total_flops / (us * 1e-6) / 1e12,
+                                  "hbm_frac": total_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                                  "units": bt.n_units, "workgroups": bt.n_workgroups}
+        bt.run()
+        torch.cuda.synchronize()
+        for e in entries:
+            A = e["case"].A
+            ref = O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B)
+            rel = float(np.linalg.norm(e["C"].cpu().numpy() - ref) / np.linalg.norm(ref))
+            assert rel <= 1e-5, f"batched {A.name}: rel-L2 {rel}"
+
+    if "fwdbwd" in args.sections:
+        extra["fwd_bwd_12_layers"] = bench_fwd_bwd(torch, np, T, O, dev)
+    if "precision" in args.sections:
+        extra["precision_4096"] = bench_precision(torch, np, T, O, dev)
 
     cpu = None
     if not args.no_cpu:
