@@ -190,8 +190,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-libs", action="store_true", help="skip rocSPARSE / hipBLASLt")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy 10x (HBM counter calibration)")
     args = ap.parse_args()
     args.sections = set(s for s in args.sections.split(",") if s)
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 256 << 20, "launches": 10}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described in DESIGN.md and is not run.", file=sys.stderr)

@@ -55,6 +55,16 @@ tca_status tca_smtx_read(const char *path, int *n_rows, int *n_cols, int *nnz,
 tca_status tca_smtx_write(const char *path, int n_rows, int n_cols, int nnz,
                           const int32_t *row_offset, const int32_t *col_idx);
 
+/* MatrixMarket coordinate text.  Replaces mtx_input_to_Dense / mtx_input_to_CSR
+ * (src/file_input.cu:8-134, flags from src/arg_parse.h:82-89): lines starting
+ * with '%' are skipped, then "rows cols nnz", then one "r c [v]" per entry.
+ * one_based subtracts 1 from r and c; pattern_only (or a missing value) gives
+ * 1.0.  The CSR comes back with columns sorted per row; a repeated (r, c)
+ * keeps the last value, as the dense reader's overwrite does.  values are
+ * fp16 bits (round to nearest even).  All arrays malloc'ed. */
+tca_status tca_mtx_read(const char *path, int one_based, int pattern_only, int *n_rows, int *n_cols, int *nnz,
+                        int32_t **row_offset, int32_t **col_idx, uint16_t **values);
+
 /* ------------------------------------------------------------------------ */
 /* Dense <-> CSR (fp16 bits).  Replaces convert_dense_to_csr /               */
 /* convert_csr_to_dense (src/miscalleneous.cu:29-65, :66-82).  With          */

@@ -9,10 +9,11 @@
 using tca::fail;
 
 struct tca_batch {
-    int n_items = 0, n_units = 0, grid = 0, device = 0;
-    void *d_mem = nullptr;  // items | units | counters
+    int n_items = 0, n_units = 0, grid = 0, device = 0, n_zero = 0;
+    void *d_mem = nullptr;  // items | units | zero list | counters
     tca::BatchItemDev *d_items = nullptr;
     tca::BatchUnitDev *d_units = nullptr;
+    tca::ZeroDesc *d_zero = nullptr;
     int *d_counters = nullptr;
 };
 
@@ -64,7 +65,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 const int g = t.work_order[gi];
                 const int nch = t.grp_chunk_ptr[g + 1] - t.grp_chunk_ptr[g];
                 for (int tt = 0; tt < d.n_tiles; tt++) {
-                    units.push_back({i, g, tt, 0});
+                    units.push_back({i, g, tt, t.grp_chunk_ptr[g], t.grp_chunk_ptr[g + 1], 0, 0, 0});
                     cost.push_back(nch * (2.0 + ppg * tile / 16.0) + 4.0);   // ~ MFMAs + staging per chunk
                 }
             }
@@ -81,7 +82,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 const int g = t.work_order[gi];
                 for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {
                     const int nc = std::min(tca::kBatchSddmmCB, t.grp_chunk_ptr[g + 1] - c);
-                    units.push_back({i, g, c, 0});
+                    units.push_back({i, g, c, 0, 0, 0, 0, 0});
                     cost.push_back(nc * 2.0 * ppg * (it.N / 32.0) + 4.0);
                 }
             }
@@ -89,6 +90,48 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: unknown op");
         }
     }
+    // Split SpMM units much longer than the average share of a workgroup into
+    // chunk-range pieces (>= 4 chunks); their items accumulate into a zeroed output.
+    int dev_id = 0, cus = 0;
+    TCA_HIP_TRY(hipGetDevice(&dev_id));
+    TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
+    const int workers = std::max(cus, 1) * 8;
+    double total_cost = 0.0;
+    for (double c : cost) total_cost += c;
+    const double share = total_cost / workers;
+    std::vector<char> item_split(n_items, 0);
+    {
+        std::vector<tca::BatchUnitDev> u2;
+        std::vector<double> c2;
+        for (size_t k = 0; k < units.size(); k++) {
+            const tca::BatchUnitDev &un = units[k];
+            const bool is_spmm = items[un.item].op == TCA_OP_SPMM;
+            const int nch = un.c_hi - un.c_lo;
+            int P = 1;
+            if (is_spmm && cost[k] > 2.0 * share) P = std::min((int)(cost[k] / share), nch / 4);
+            if (P <= 1) {
+                u2.push_back(un);
+                c2.push_back(cost[k]);
+                continue;
+            }
+            item_split[un.item] = 1;
+            for (int q = 0; q < P; q++) {
+                tca::BatchUnitDev v = un;
+                v.c_lo = un.c_lo + (int)((long)nch * q / P);
+                v.c_hi = un.c_lo + (int)((long)nch * (q + 1) / P);
+                u2.push_back(v);
+                c2.push_back(cost[k] / P);
+            }
+        }
+        units.swap(u2);
+        cost.swap(c2);
+    }
+    std::vector<tca::ZeroDesc> zero;
+    for (int i = 0; i < n_items; i++)
+        if (item_split[i]) zero.push_back({static_cast<float *>(items[i].out), dev[i].n_rows, items[i].N, items[i].ldc, 0});
+    for (auto &un : units)
+        if (item_split[un.item]) un.atomic = 1;
+
     // heaviest units first
     std::vector<int> order(units.size());
     std::iota(order.begin(), order.end(), 0);
@@ -100,21 +143,24 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     if (!b) return fail(TCA_ERR_ALLOC, "tca_batch_create: host allocation");
     b->n_items = n_items;
     b->n_units = (int)sorted.size();
-    TCA_HIP_TRY(hipGetDevice(&b->device));
-    int cus = 0;
-    TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    b->grid = std::max(1, std::min(b->n_units, std::max(cus, 1) * 8));
+    b->n_zero = (int)zero.size();
+    b->device = dev_id;
+    b->grid = std::max(1, std::min(b->n_units, workers));
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
-    const size_t total = bi + bu + 256;
+    const size_t bz = sizeof(tca::ZeroDesc) * std::max<size_t>(zero.size(), 1);
+    const size_t total = bi + bu + bz + 256;
     if (hipMalloc(&b->d_mem, total) != hipSuccess) { delete b; return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation"); }
     char *m = static_cast<char *>(b->d_mem);
     b->d_items = reinterpret_cast<tca::BatchItemDev *>(m);
     b->d_units = reinterpret_cast<tca::BatchUnitDev *>(m + bi);
-    b->d_counters = reinterpret_cast<int *>(m + bi + bu);
+    b->d_zero = reinterpret_cast<tca::ZeroDesc *>(m + bi + bu);
+    b->d_counters = reinterpret_cast<int *>(m + bi + bu + bz);
     if (hipMemcpy(b->d_items, dev.data(), bi, hipMemcpyHostToDevice) != hipSuccess ||
         (!sorted.empty() && hipMemcpy(b->d_units, sorted.data(), sizeof(tca::BatchUnitDev) * sorted.size(),
                                       hipMemcpyHostToDevice) != hipSuccess) ||
+        (!zero.empty() && hipMemcpy(b->d_zero, zero.data(), sizeof(tca::ZeroDesc) * zero.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemset(b->d_counters, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         tca_batch_destroy(b);
         return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
@@ -127,7 +173,7 @@ extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
 {
     if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
     return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->grid,
-                             static_cast<hipStream_t>(stream));
+                             batch->d_zero, batch->n_zero, static_cast<hipStream_t>(stream));
 }
 
 extern "C" tca_status tca_batch_destroy(tca_batch *batch)

@@ -145,10 +145,12 @@ extern "C" tca_status tca_plan_destroy(tca_plan *plan)
     if (!plan) return TCA_OK;
     int cur = 0;
     bool switched = false;
-    if ((plan->d_base || plan->d_sddmm_units) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
+    if ((plan->d_base || plan->d_sddmm_units || !plan->unit_tables.empty()) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
         switched = hipSetDevice(plan->device) == hipSuccess;
     if (plan->d_base) (void)hipFree(plan->d_base);
     if (plan->d_sddmm_units) (void)hipFree(plan->d_sddmm_units);
+    for (auto &u : plan->unit_tables)
+        if (u.dev) (void)hipFree(u.dev);
     if (switched) (void)hipSetDevice(cur);
     delete plan;
     return TCA_OK;

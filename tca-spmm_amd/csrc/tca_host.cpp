@@ -181,6 +181,102 @@ extern "C" tca_status tca_smtx_write(const char *path, int n_rows, int n_cols, i
 }
 
 // ---------------------------------------------------------------------------
+// MatrixMarket coordinate reader (src/file_input.cu:8-134)
+// ---------------------------------------------------------------------------
+namespace {
+uint16_t f32_to_f16_rne(float f);
+}
+
+extern "C" tca_status tca_mtx_read(const char *path, int one_based, int pattern_only, int *n_rows, int *n_cols,
+                                   int *nnz, int32_t **row_offset, int32_t **col_idx, uint16_t **values)
+{
+    if (!path || !n_rows || !n_cols || !nnz || !row_offset || !col_idx || !values)
+        return fail(TCA_ERR_INVALID_ARG, "tca_mtx_read: NULL argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(TCA_ERR_IO, std::string("tca_mtx_read: cannot open ") + path);
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    std::fclose(f);
+
+    struct Ent { int32_t r, c; float v; int32_t seq; };
+    std::vector<Ent> ents;
+    long M = -1, K = -1, NZ = -1;
+    size_t pos = 0;
+    int seq = 0;
+    while (pos < text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        const std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        size_t first = line.find_first_not_of(" \t\r");
+        if (first == std::string::npos) continue;     // blank line
+        if (line[first] == '%') continue;             // comment / banner
+        const char *p = line.c_str() + first;
+        char *end;
+        if (M < 0) {
+            M = std::strtol(p, &end, 10);
+            if (end == p) return fail(TCA_ERR_FORMAT, "mtx: bad header");
+            p = end;
+            K = std::strtol(p, &end, 10);
+            if (end == p) return fail(TCA_ERR_FORMAT, "mtx: bad header");
+            p = end;
+            NZ = std::strtol(p, &end, 10);
+            if (end == p) return fail(TCA_ERR_FORMAT, "mtx: bad header");
+            if (M <= 0 || K <= 0 || NZ < 0 || M > (1L << 28) || K > (1L << 28) || NZ > 0x7fffffffL)
+                return fail(TCA_ERR_FORMAT, "mtx: header values out of range");
+            continue;
+        }
+        long r = std::strtol(p, &end, 10);
+        if (end == p) return fail(TCA_ERR_FORMAT, "mtx: bad entry line");
+        p = end;
+        long c = std::strtol(p, &end, 10);
+        if (end == p) return fail(TCA_ERR_FORMAT, "mtx: bad entry line");
+        p = end;
+        float v = 1.0f;
+        if (!pattern_only) {
+            const float x = std::strtof(p, &end);
+            if (end != p) v = x;
+        }
+        if (one_based) { r--; c--; }
+        if (r < 0 || r >= M || c < 0 || c >= K) return fail(TCA_ERR_FORMAT, "mtx: entry index out of range");
+        ents.push_back({(int32_t)r, (int32_t)c, v, seq++});
+    }
+    if (M < 0) return fail(TCA_ERR_FORMAT, "mtx: missing header");
+    if ((long)ents.size() != NZ) return fail(TCA_ERR_FORMAT, "mtx: entry count differs from the header");
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        return a.r != b.r ? a.r < b.r : a.c < b.c;
+    });
+    // repeated (r, c): the last one in file order wins (stable sort keeps file order)
+    std::vector<Ent> uniq;
+    for (const Ent &e : ents) {
+        if (!uniq.empty() && uniq.back().r == e.r && uniq.back().c == e.c) uniq.back() = e;
+        else uniq.push_back(e);
+    }
+    int32_t *rp = static_cast<int32_t *>(std::calloc((size_t)M + 1, sizeof(int32_t)));
+    int32_t *ci = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * (uniq.size() + 1)));
+    uint16_t *vv = static_cast<uint16_t *>(std::malloc(sizeof(uint16_t) * (uniq.size() + 1)));
+    if (!rp || !ci || !vv) {
+        std::free(rp); std::free(ci); std::free(vv);
+        return fail(TCA_ERR_ALLOC, "mtx: CSR arrays");
+    }
+    for (size_t k = 0; k < uniq.size(); k++) {
+        rp[uniq[k].r + 1]++;
+        ci[k] = uniq[k].c;
+        vv[k] = f32_to_f16_rne(uniq[k].v);
+    }
+    for (long r = 0; r < M; r++) rp[r + 1] += rp[r];
+    *n_rows = (int)M;
+    *n_cols = (int)K;
+    *nnz = (int)uniq.size();
+    *row_offset = rp;
+    *col_idx = ci;
+    *values = vv;
+    return TCA_OK;
+}
+
+// ---------------------------------------------------------------------------
 // dense <-> CSR on fp16 bits (src/miscalleneous.cu:29-65, :66-82)
 // A half is nonzero iff its float value != 0.0f: +-0 are dropped, NaN is kept.
 // ---------------------------------------------------------------------------

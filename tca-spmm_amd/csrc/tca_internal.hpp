@@ -59,6 +59,13 @@ struct tca_plan {
     // SDDMM work units (group, first chunk), built on first use
     void *d_sddmm_units = nullptr;
     int n_sddmm_units = 0;
+    // SpMM unit tables (group, pieces, c_lo, c_hi), one per split factor
+    struct UnitTable {
+        int pieces = 1, count = 0;
+        bool split = false;
+        void *dev = nullptr;
+    };
+    std::vector<UnitTable> unit_tables;
     // CSR pattern kept for building the transpose
     std::vector<int32_t> csr_row_offset, csr_col_idx;
 };
@@ -91,12 +98,18 @@ struct BatchItemDev {
     float *out;
 };
 struct BatchUnitDev {
-    int32_t item, g, t, pad;   // SPMM: (group, N-tile); SDDMM: (group, first chunk)
+    int32_t item, g, t;        // SPMM: (group, N-tile); SDDMM: (group, first chunk)
+    int32_t c_lo, c_hi;        // SPMM: chunk range of the group
+    int32_t atomic, pad0, pad1;
+};
+struct ZeroDesc {              // an output that split units accumulate into
+    float *C;
+    int32_t rows, N, ldc, pad;
 };
 constexpr int kBatchSddmmCB = 4;
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed);
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        hipStream_t stream);
+                        const ZeroDesc *zero, int n_zero, hipStream_t stream);
 
 }  // namespace tca

@@ -136,7 +136,8 @@ struct SpmmParams {
     float *C;
     const float *row_scale;
     const float *col_scale;
-    int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw;
+    const int4 *units;       // standalone launch: (group, piece, c_lo, c_hi) per workgroup row
+    int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw, split;
 };
 
 template <int ES> struct Store;
@@ -167,10 +168,21 @@ __device__ inline typename Store<KPL * ES>::T pack_k(const Oct (&rows)[KPL], int
     return out;
 }
 
-// One wave = one (group g, N-tile `tile`).  RPW rows (PPG = RPW/16 panels) x NT
-// columns.  lds_b holds NT*32 elements, lds_a PPG*512 elements.
+// Entries prefetched per lane per panel and chunk (covers 128 entries of a
+// 16x32 panel tile; denser tiles take an extra, unprefetched pass).
+constexpr int kEP = 2;
+
+// One wave = one (group g, N-tile `tile`, chunks [c_lo, c_hi) of g).  RPW rows
+// (PPG = RPW/16 panels) x NT columns; lds_b holds NT*32 elements, lds_a
+// PPG*512 elements.  Software pipeline, one chunk per iteration:
+//   stage A (two chunks ahead): column ids and segment bounds,
+//   stage B (one chunk ahead):  gathered B rows and the panel entries,
+//   stage C (this chunk):       LDS transpose + scatter, MFMAs, un-scatter.
+// atomic: several waves share the output tile (chunk ranges split a group);
+// the caller has zeroed C and the wave adds its partial sums.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, uint8_t *lds_b, uint8_t *lds_a)
+__device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, bool atomic,
+                                          uint8_t *lds_b, uint8_t *lds_a)
 {
     constexpr int PPG = RPW / 16;
     constexpr int ES = Elt<AT>::ES;
@@ -184,7 +196,6 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
 
     const int lane = threadIdx.x;
     const int n0 = tile * NT;
-    const int c_begin = p.grp_chunk_ptr[g], c_end = p.grp_chunk_ptr[g + 1];
 
     f32x4 acc[PPG][NSUB];
 #pragma unroll
@@ -197,44 +208,104 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     const int ncol = n0 + 8 * seg;
     const bool col_ok = ncol < p.N;
     const E *avals = static_cast<const E *>(p.a);
+    const int seg_lane = lane <= PPG ? lane : PPG;
 
-    for (int c = c_begin; c < c_end; ++c) {
-        // 1. gather KPL condensed B rows (8 columns each) into registers
-        Oct rows[KPL];
+#pragma unroll
+    for (int i = 0; i < PPG; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+
+    // ---- pipeline registers
+    int cols_n[KPL];               // column ids of chunk c+1
+    int segb_n = 0;                // lane j <= PPG: seg_ptr[(c+1)*PPG + j]
+    Oct rows_c[KPL];               // B rows of chunk c
+    int ent_c[PPG][kEP];           // entry index of chunk c (-1: none)
+    int slot_c[PPG][kEP];
+    uint32_t val_c[PPG][kEP];      // value bits (non-indirect) / entry_src (indirect)
+    int e0_c[PPG], e1_c[PPG];
+
+    auto load_index = [&](int c, int (&cols)[KPL], int &segb) {
         const int32_t *cc = p.chunk_cols + (long)c * 32 + k0;
 #pragma unroll
-        for (int j = 0; j < KPL; j++) {
-            const int col = cc[j];
-            rows[j] = (col >= 0 && col_ok) ? RowLoad<AT, BT>::load(p.B, (long)col * p.ldb + ncol) : zero_oct<Oct>();
-        }
-        // 2. zero the A tiles, then scatter this chunk's entries
+        for (int j = 0; j < KPL; j++) cols[j] = cc[j];
+        segb = p.seg_ptr[c * PPG + seg_lane];
+    };
+    auto load_data = [&](const int (&cols)[KPL], int segb, Oct (&rows)[KPL], int (&ent)[PPG][kEP], int (&slot)[PPG][kEP],
+                         uint32_t (&val)[PPG][kEP], int (&e0)[PPG], int (&e1)[PPG]) {
 #pragma unroll
-        for (int i = 0; i < PPG; i++)
-            reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
-        __syncthreads();
+        for (int j = 0; j < KPL; j++)
+            rows[j] = (cols[j] >= 0 && col_ok) ? RowLoad<AT, BT>::load(p.B, (long)cols[j] * p.ldb + ncol) : zero_oct<Oct>();
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
-            const int e0 = p.seg_ptr[c * PPG + i], e1 = p.seg_ptr[c * PPG + i + 1];
-            for (int e = e0 + lane; e < e1; e += 64) {
-                const int slot = p.entry_slot[e];
-                const E v = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                const int off = (i * 64 + a_oct(slot >> 5, (slot >> 3) & 3)) * OCT + (slot & 7) * ES;
-                *reinterpret_cast<E *>(lds_a + off) = v;
+            e0[i] = __shfl(segb, i);
+            e1[i] = __shfl(segb, i + 1);
+#pragma unroll
+            for (int k = 0; k < kEP; k++) {
+                const int e = e0[i] + k * 64 + lane;
+                const bool ok = e < e1[i];
+                ent[i][k] = ok ? e : -1;
+                slot[i][k] = ok ? (int)p.entry_slot[e] : 0;
+                if (INDIRECT) val[i][k] = ok ? (uint32_t)p.entry_src[e] : 0u;
+                else val[i][k] = ok ? (uint32_t)avals[e] : 0u;
             }
         }
-        // 3. transposed write of the gathered rows: LDS[n][k0 .. k0+KPL)
+    };
+
+    if (c_lo < c_hi) {
+        int cols_c[KPL], segb_c;
+        load_index(c_lo, cols_c, segb_c);
+        load_data(cols_c, segb_c, rows_c, ent_c, slot_c, val_c, e0_c, e1_c);
+        if (c_lo + 1 < c_hi) load_index(c_lo + 1, cols_n, segb_n);
+    }
+
+    for (int c = c_lo; c < c_hi; ++c) {
+        // stage B for c+1 and stage A for c+2 (issued before this chunk's work)
+        Oct rows_n[KPL];
+        int ent_n[PPG][kEP], slot_n[PPG][kEP], e0_n[PPG], e1_n[PPG];
+        uint32_t val_n[PPG][kEP];
+        int cols_nn[KPL], segb_nn = 0;
+#pragma unroll
+        for (int j = 0; j < KPL; j++) cols_nn[j] = -1;
+        const bool has_n = c + 1 < c_hi;
+        if (has_n) {
+            load_data(cols_n, segb_n, rows_n, ent_n, slot_n, val_n, e0_n, e1_n);
+            if (c + 2 < c_hi) load_index(c + 2, cols_nn, segb_nn);
+        }
+        // values of this chunk (indirect: through the forward tiled order)
+        E v_c[PPG][kEP];
+#pragma unroll
+        for (int i = 0; i < PPG; i++)
+#pragma unroll
+            for (int k = 0; k < kEP; k++)
+                v_c[i][k] = INDIRECT ? (ent_c[i][k] >= 0 ? avals[val_c[i][k]] : (E)0) : (E)val_c[i][k];
+
+        // stage C: transposed B tile, A scatter
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int n = 8 * seg + e;
             const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
-            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
+            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows_c, e);
+        }
+        bool overflow[PPG];
+#pragma unroll
+        for (int i = 0; i < PPG; i++) {
+#pragma unroll
+            for (int k = 0; k < kEP; k++)
+                if (ent_c[i][k] >= 0) {
+                    const int s = slot_c[i][k];
+                    *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = v_c[i][k];
+                }
+            overflow[i] = e1_c[i] - e0_c[i] > kEP * 64;
+            if (overflow[i]) {
+                for (int e = e0_c[i] + kEP * 64 + lane; e < e1_c[i]; e += 64) {
+                    const int s = p.entry_slot[e];
+                    const E v = INDIRECT ? avals[p.entry_src[e]] : avals[e];
+                    *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = v;
+                }
+            }
         }
         __syncthreads();
-        // 4. operands and MFMAs
         Oct a[PPG];
 #pragma unroll
-        for (int i = 0; i < PPG; i++)
-            a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
+        for (int i = 0; i < PPG; i++) a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
             const Oct b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
@@ -242,9 +313,37 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
         }
         __syncthreads();
+        // un-scatter: put the zeros back (a whole tile when it overflowed)
+#pragma unroll
+        for (int i = 0; i < PPG; i++) {
+            if (overflow[i]) {
+                reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+            } else {
+#pragma unroll
+                for (int k = 0; k < kEP; k++)
+                    if (ent_c[i][k] >= 0) {
+                        const int s = slot_c[i][k];
+                        *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = (E)0;
+                    }
+            }
+        }
+        // rotate the pipeline
+        if (has_n) {
+#pragma unroll
+            for (int j = 0; j < KPL; j++) { rows_c[j] = rows_n[j]; cols_n[j] = cols_nn[j]; }
+#pragma unroll
+            for (int i = 0; i < PPG; i++) {
+                e0_c[i] = e0_n[i];
+                e1_c[i] = e1_n[i];
+#pragma unroll
+                for (int k = 0; k < kEP; k++) { ent_c[i][k] = ent_n[i][k]; slot_c[i][k] = slot_n[i][k]; val_c[i][k] = val_n[i][k]; }
+            }
+            segb_n = segb_nn;
+        }
     }
+    __syncthreads();  // LDS is clean for the next unit
 
-    // 5. epilogue: lane holds D[4*(lane/16)+j][lane%16] of each 16x16 tile
+    // epilogue: lane holds D[4*(lane/16)+j][lane%16] of each 16x16 tile
     const int rbase = g * RPW + 4 * (lane >> 4);
 #pragma unroll
     for (int i = 0; i < PPG; i++) {
@@ -260,21 +359,24 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
                 if (n < p.N) {
                     float v = acc[i][s][j] * rs;
                     if (p.col_scale) v *= p.col_scale[n];
-                    crow[n] = v;
+                    if (atomic) unsafeAtomicAdd(crow + n, v);
+                    else crow[n] = v;
                 }
             }
         }
     }
 }
 
+// Standalone launch: workgroup b runs unit b of the plan's unit table
+// (group, N-tile, chunk range), heavy groups first.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
 __global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds_b[NT * 32 * Elt<AT>::ES];
     __shared__ __attribute__((aligned(16))) uint8_t lds_a[(RPW / 16) * 512 * Elt<AT>::ES];
-    const int gi = blockIdx.x / p.n_tiles;
-    const int tile = blockIdx.x - gi * p.n_tiles;
-    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, p.work_order[gi], tile, lds_b, lds_a);
+    const int4 u = p.units[blockIdx.x];   // (group, split piece, c_lo, c_hi)
+    const int tile = blockIdx.y;
+    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, u.x, tile, u.z, u.w, p.split, lds_b, lds_a);
 }
 
 using SpmmFn = void (*)(SpmmParams);
@@ -422,6 +524,14 @@ __global__ void gather_kernel(const int32_t *__restrict__ idx, int n, const T *_
     }
 }
 
+// C[0:rows, 0:N] = 0 (before split-chunk launches accumulate into it)
+__global__ void zero_rows_kernel(float *C, int rows, int N, int ldc)
+{
+    const long total = (long)rows * N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
+        C[(i / N) * ldc + (i % N)] = 0.0f;
+}
+
 // f32 -> OCP E4M3FN, round to nearest even, saturating to +-448; the sign is
 // always kept (tiny negatives give 0x80).
 __device__ inline uint8_t f32_to_e4m3(float x)
@@ -515,6 +625,8 @@ __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
     p.C = d.out;
     p.row_scale = nullptr;
     p.col_scale = nullptr;
+    p.units = nullptr;
+    p.split = 0;
     p.n_rows = d.n_rows;
     p.N = d.N;
     p.ldb = d.ldb;
@@ -546,10 +658,16 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     return p;
 }
 
-#define TCA_FWD_CASE(id, R, T) \
-    case id: spmm_unit<R, T, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, lds_b, lds_a); break;
-#define TCA_BWD_CASE(id, R, T) \
-    case id: spmm_unit<R, T, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, lds_b, lds_a); break;
+#define TCA_FWD_CASE(id, R, T)                                                                          \
+    case id:                                                                                            \
+        spmm_unit<R, T, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, \
+                                                 lds_b, lds_a);                                         \
+        break;
+#define TCA_BWD_CASE(id, R, T)                                                                          \
+    case id:                                                                                            \
+        spmm_unit<R, T, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, \
+                                                lds_b, lds_a);                                          \
+        break;
 
 __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                    const BatchUnitDev *__restrict__ units, int n_units, int *counters)
@@ -592,6 +710,15 @@ __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restric
 #undef TCA_FWD_CASE
 #undef TCA_BWD_CASE
 
+// zero the outputs of split batch items (blockIdx.y = descriptor)
+__global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
+{
+    const ZeroDesc z = list[blockIdx.y];
+    const long total = (long)z.rows * z.N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
+        z.C[(i / z.N) * z.ldc + (i % z.N)] = 0.0f;
+}
+
 }  // namespace
 
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
@@ -613,8 +740,12 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
 }
 
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        hipStream_t stream)
+                        const ZeroDesc *zero, int n_zero, hipStream_t stream)
 {
+    if (n_zero > 0) {
+        hipLaunchKernelGGL(zero_list_kernel, dim3(256, n_zero), dim3(256), 0, stream, zero);
+        TCA_HIP_TRY(hipGetLastError());
+    }
     if (n_units <= 0) return TCA_OK;
     hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(64), 0, stream, items, units, n_units, counters);
     TCA_HIP_TRY(hipGetLastError());
@@ -640,6 +771,51 @@ int default_tile_n(const tca_plan *p, int N)
     if (N >= 128) return 64;
     if (N >= 32) return 32;
     return 16;
+}
+
+// Chunk ranges per group: with few (group, N-tile) waves a group's chunks are
+// split into pieces so that roughly kTargetWaves waves run; pieces keep at
+// least kMinPieceChunks chunks.  Returns the device unit table for the plan.
+static constexpr int kTargetWaves = 2048;
+static constexpr int kMinPieceChunks = 4;
+
+static tca_status spmm_units(const tca_plan *cp, int n_tiles, const int4 **units, int *n_units, bool *split)
+{
+    tca_plan *p = const_cast<tca_plan *>(cp);
+    const PlanTables &t = p->host;
+    const long waves = (long)t.n_groups * n_tiles;
+    int S = 1;
+    if (waves > 0 && waves < kTargetWaves / 2) S = (int)std::min<long>(16, (kTargetWaves + waves - 1) / waves);
+    for (auto &u : p->unit_tables)
+        if (u.pieces == S) {
+            *units = static_cast<const int4 *>(u.dev);
+            *n_units = u.count;
+            *split = u.split;
+            return TCA_OK;
+        }
+    std::vector<int4> h;
+    bool any_split = false;
+    for (int gi = 0; gi < t.n_groups; gi++) {
+        const int g = t.work_order[gi];
+        const int cb = t.grp_chunk_ptr[g], nch = t.grp_chunk_ptr[g + 1] - cb;
+        const int P = std::max(1, std::min(S, nch / kMinPieceChunks));
+        if (P > 1) any_split = true;
+        for (int k = 0; k < P; k++)
+            h.push_back(make_int4(g, P, cb + (int)((long)nch * k / P), cb + (int)((long)nch * (k + 1) / P)));
+    }
+    tca_plan::UnitTable ut;
+    ut.pieces = S;
+    ut.count = (int)h.size();
+    ut.split = any_split;
+    if (!h.empty()) {
+        TCA_HIP_TRY(hipMalloc(&ut.dev, sizeof(int4) * h.size()));
+        TCA_HIP_TRY(hipMemcpy(ut.dev, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    }
+    p->unit_tables.push_back(ut);
+    *units = static_cast<const int4 *>(ut.dev);
+    *n_units = ut.count;
+    *split = ut.split;
+    return TCA_OK;
 }
 
 tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_dtype b_dtype, const void *B, int ldb,
@@ -670,10 +846,28 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.n_tiles = (N + tile_n - 1) / tile_n;
     prm.n_groups = t.n_groups;
     prm.rpw = t.rpw;
-    const long blocks = (long)t.n_groups * prm.n_tiles;
-    if (blocks <= 0) return TCA_OK;
-    if (blocks > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: grid too large");
-    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64), 0, stream, prm);
+    int n_units = 0;
+    bool split = false;
+    tca_status st = spmm_units(p, prm.n_tiles, &prm.units, &n_units, &split);
+    if (st != TCA_OK) return st;
+    prm.split = split ? 1 : 0;
+    if (split) {  // pieces accumulate: start from zero (a kernel, so graph capture stays memset-free)
+        const long total = (long)t.n_rows * N;
+        const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(zero_rows_kernel, dim3(blocks), dim3(256), 0, stream, C, t.n_rows, N, ldc);
+        TCA_HIP_TRY(hipGetLastError());
+    }
+    if (n_units <= 0) {
+        if (!split && t.n_rows > 0) {  // no nonzeros at all: C = 0
+            const long total = (long)t.n_rows * N;
+            const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+            hipLaunchKernelGGL(zero_rows_kernel, dim3(blocks), dim3(256), 0, stream, C, t.n_rows, N, ldc);
+            TCA_HIP_TRY(hipGetLastError());
+        }
+        return TCA_OK;
+    }
+    if (prm.n_tiles > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: too many N tiles");
+    hipLaunchKernelGGL(fn, dim3((unsigned)n_units, (unsigned)prm.n_tiles), dim3(64), 0, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
