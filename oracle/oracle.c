@@ -196,6 +196,45 @@ int orc_smtx_to_dense(const char *path, int *M, int *K, int *nnz, uint16_t **den
 }
 
 /* ------------------------------------------------------------------------ */
+/* MatrixMarket -> dense -- mtx_input_to_Dense, src/file_input.cu:8-61       */
+/* ------------------------------------------------------------------------ */
+int orc_mtx_to_dense(const char *path, int one_based, int *M, int *K, int *nnz, uint16_t **dense)
+{
+    size_t len = 0;
+    char *buf = slurp(path, &len);
+    if (!buf) return -1;
+    long rows = -1, cols = -1, meta = -1, count = 0;
+    uint16_t *d = NULL;
+    char *save = NULL;
+    for (char *line = strtok_r(buf, "\n", &save); line; line = strtok_r(NULL, "\n", &save)) {
+        char *s = line;
+        while (*s == ' ' || *s == '\t' || *s == '\r') s++;
+        if (*s == 0 || *s == '%') continue;
+        if (rows < 0) {
+            if (sscanf(s, "%ld %ld %ld", &rows, &cols, &meta) != 3 || rows <= 0 || cols <= 0 || meta < 0) {
+                free(buf);
+                return -2;
+            }
+            d = (uint16_t *)calloc((size_t)rows * cols, sizeof(uint16_t));
+            continue;
+        }
+        long r, c;
+        float v = 1.0f;
+        int got = sscanf(s, "%ld %ld %f", &r, &c, &v);
+        if (got < 2) { free(d); free(buf); return -3; }
+        if (one_based) { r--; c--; }
+        if (r < 0 || r >= rows || c < 0 || c >= cols) { free(d); free(buf); return -4; }
+        d[(size_t)r * cols + c] = orc_f32_to_f16(v);
+        count++;
+    }
+    free(buf);
+    if (rows < 0) return -2;
+    if (count != meta) { free(d); return -5; }
+    *M = (int)rows; *K = (int)cols; *nnz = (int)meta; *dense = d;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* dense <-> CSR -- src/miscalleneous.cu:29-65 and :66-82                    */
 /* ------------------------------------------------------------------------ */
 int orc_dense_to_csr(const uint16_t *dense, int M, int K, int *rowptr, int *colidx, uint16_t *vals)

@@ -47,6 +47,12 @@ uint8_t  orc_f32_to_e4m3(float f);         /* nearest even, saturate to +-448, N
  * 1.0 (0x3C00) at every pattern position (src/file_input.cu:189). */
 int orc_smtx_to_dense(const char *path, int *M, int *K, int *nnz, uint16_t **dense);
 
+/* ---- MatrixMarket reader: mtx_input_to_Dense, src/file_input.cu:8-61 ------ */
+/* '%' lines skipped, header "rows cols nnz", entries "r c [v]" (v defaults to
+ * 1.0), minus one when one_based; later duplicates overwrite.  Returns 0, or
+ * <0 on a malformed file (the reference asserts on the count). */
+int orc_mtx_to_dense(const char *path, int one_based, int *M, int *K, int *nnz, uint16_t **dense);
+
 /* ---- dense <-> CSR: src/miscalleneous.cu:29-65, :66-82 -------------------- */
 /* Counts entries with (float)x != 0.0f; if colidx/vals are non-NULL fills them
  * (row-major scan, sorted columns).  Returns nnz. */

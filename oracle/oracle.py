@@ -47,6 +47,9 @@ def lib():
         L.orc_smtx_to_dense.restype = ctypes.c_int
         L.orc_smtx_to_dense.argtypes = [ctypes.c_char_p, _i32p, _i32p, _i32p,
                                         ctypes.POINTER(ctypes.POINTER(ctypes.c_uint16))]
+        L.orc_mtx_to_dense.restype = ctypes.c_int
+        L.orc_mtx_to_dense.argtypes = [ctypes.c_char_p, ctypes.c_int, _i32p, _i32p, _i32p,
+                                       ctypes.POINTER(ctypes.POINTER(ctypes.c_uint16))]
         L.orc_dense_to_csr.restype = ctypes.c_int
         L.orc_dense_to_csr.argtypes = [_u16p, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _u16p]
         L.orc_csr_to_dense.restype = None
@@ -138,6 +141,19 @@ def smtx_to_dense(path):
     M, K, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     ptr = ctypes.POINTER(ctypes.c_uint16)()
     rc = L.orc_smtx_to_dense(path.encode(), ctypes.byref(M), ctypes.byref(K), ctypes.byref(nnz), ctypes.byref(ptr))
+    if rc != 0:
+        return rc, None
+    arr = np.ctypeslib.as_array(ptr, shape=(M.value * K.value,)).copy().reshape(M.value, K.value)
+    L.orc_free(ptr)
+    return 0, (arr, nnz.value)
+
+
+def mtx_to_dense(path, one_based=True):
+    L = lib()
+    M, K, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    ptr = ctypes.POINTER(ctypes.c_uint16)()
+    rc = L.orc_mtx_to_dense(path.encode(), int(one_based), ctypes.byref(M), ctypes.byref(K), ctypes.byref(nnz),
+                            ctypes.byref(ptr))
     if rc != 0:
         return rc, None
     arr = np.ctypeslib.as_array(ptr, shape=(M.value * K.value,)).copy().reshape(M.value, K.value)

@@ -58,6 +58,8 @@ SIGNATURES = {
     "tca_smtx_read": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                            ctypes.POINTER(_i32p), ctypes.POINTER(_i32p)]),
     "tca_smtx_write": (_i, [ctypes.c_char_p, _i, _i, _i, _i32p, _i32p]),
+    "tca_mtx_read": (_i, [ctypes.c_char_p, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                          ctypes.POINTER(_i32p), ctypes.POINTER(_i32p), ctypes.POINTER(_u16p)]),
     "tca_dense_to_csr_f16": (_i, [_u16p, _i, _i, _i32p, _i32p, _u16p, _i, ctypes.POINTER(_i)]),
     "tca_csr_to_dense_f16": (_i, [_i, _i, _i32p, _i32p, _u16p, _u16p]),
     "tca_compare_mat_f32": (_i, [_f32p, _f32p, _i, ctypes.c_float, ctypes.POINTER(_i)]),

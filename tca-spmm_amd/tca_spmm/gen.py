@@ -74,6 +74,20 @@ def smtx_read(path):
     return M.value, K.value, row_offset, col_idx
 
 
+def mtx_read(path, one_based=True, pattern_only=False):
+    """MatrixMarket coordinate file -> (n_rows, n_cols, row_offset, col_idx, fp16 value bits)."""
+    M, K, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rp, ci, vv = _i32p(), _i32p(), _u16p()
+    call("tca_mtx_read", path.encode(), int(one_based), int(pattern_only), ctypes.byref(M), ctypes.byref(K),
+         ctypes.byref(nnz), ctypes.byref(rp), ctypes.byref(ci), ctypes.byref(vv))
+    n = max(nnz.value, 1)
+    out = (M.value, K.value, np.ctypeslib.as_array(rp, shape=(M.value + 1,)).copy(),
+           np.ctypeslib.as_array(ci, shape=(n,))[:nnz.value].copy(), np.ctypeslib.as_array(vv, shape=(n,))[:nnz.value].copy())
+    for ptr in (rp, ci, vv):
+        _lib.lib().tca_host_free(ptr)
+    return out
+
+
 def smtx_write(path, n_rows, n_cols, row_offset, col_idx):
     rp = np.ascontiguousarray(row_offset, np.int32)
     ci = np.ascontiguousarray(col_idx, np.int32)

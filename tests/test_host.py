@@ -206,3 +206,35 @@ def test_plan_rejects_bad_csr():
     with pytest.raises(T.TcaError) as e:
         T.Plan(2, 4, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), rows_per_wave=8, device=False)
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+
+
+# ------------------------------------------------------------ MatrixMarket --
+def test_mtx_read_matches_oracle(tmp_path):
+    rng = np.random.default_rng(9)
+    M, K = 60, 45
+    rows, cols = np.nonzero(rng.random((M, K)) < 0.1)
+    perm = rng.permutation(rows.size)                       # file order is not sorted
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    lines = ["%%MatrixMarket matrix coordinate real general", "% generated", f"{M} {K} {rows.size + 1}"]
+    lines += [f"{rows[i] + 1} {cols[i] + 1} {vals[i]:.6f}" for i in perm]
+    lines.append(f"{rows[perm[0]] + 1} {cols[perm[0]] + 1} 7.5")   # duplicate: last wins
+    p = str(tmp_path / "r.mtx")
+    open(p, "w").write("\n".join(lines) + "\n")
+    M2, K2, rp, ci, v = gen.mtx_read(p, one_based=True)
+    rc, (dense, nnz) = O.mtx_to_dense(p, one_based=True)
+    assert (M2, K2) == (M, K) and rc == 0
+    assert np.array_equal(gen.csr_to_dense_f16(M, K, rp, ci, v), dense)
+    assert ci.size == rows.size                             # the duplicate is merged
+    for r in range(M):
+        assert np.all(np.diff(ci[rp[r]:rp[r + 1]]) > 0)
+    _, _, rp2, ci2, v2 = gen.mtx_read(p, one_based=True, pattern_only=True)
+    assert np.array_equal(rp2, rp) and np.array_equal(ci2, ci) and np.all(v2 == 0x3C00)
+
+
+@pytest.mark.parametrize("text", ["2 2 3\n1 1 1\n", "2 2 1\n3 1 1\n", "x y z\n1 1 1\n", ""])
+def test_mtx_read_rejects_malformed(tmp_path, text):
+    p = tmp_path / "bad.mtx"
+    p.write_text(text)
+    with pytest.raises(T.TcaError) as e:
+        gen.mtx_read(str(p))
+    assert e.value.name == "TCA_ERR_FORMAT"

@@ -297,3 +297,25 @@ def test_transpose_matches_scipy():
     AT.sort_indices()
     assert np.array_equal(rpT, AT.indptr) and np.array_equal(cT, AT.indices)
     assert np.array_equal(perm + 1, AT.data.astype(np.int64))
+
+
+# ------------------------------------------------------------ MatrixMarket --
+def test_mtx_reader_brute_force(tmp_path):
+    p = tmp_path / "a.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n% comment\n3 4 5\n"
+                 "1 1 0.5\n3 4 -2\n2 2\n1 1 0.25\n  2 3 1.5\n")
+    rc, (dense, nnz) = O.mtx_to_dense(str(p), one_based=True)
+    assert rc == 0 and nnz == 5
+    exp = np.zeros((3, 4), np.float16)
+    exp[0, 0] = 0.25          # the later duplicate overwrites
+    exp[2, 3] = -2
+    exp[1, 1] = 1.0           # missing value -> 1.0
+    exp[1, 2] = 1.5
+    assert np.array_equal(dense, exp.view(np.uint16))
+    p.write_text("2 2 2\n0 1 3\n1 0 4\n")
+    rc, (dense, _) = O.mtx_to_dense(str(p), one_based=False)
+    assert rc == 0 and dense.view(np.float16)[0, 1] == 3 and dense.view(np.float16)[1, 0] == 4
+    p.write_text("2 2 3\n1 1 1\n")
+    assert O.mtx_to_dense(str(p))[0] == -5                  # count differs from the header
+    p.write_text("2 2 1\n3 1 1\n")
+    assert O.mtx_to_dense(str(p))[0] == -4                  # index out of range
