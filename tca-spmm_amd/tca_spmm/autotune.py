@@ -52,9 +52,10 @@ def choice_for(M, K, nnz, N, op="spmm"):
     return heuristic(M, K, nnz, N)
 
 
-def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda"):
+def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False):
     """Time each (rows_per_wave, tile_n) on the device.  A: workloads.SparseMatrix
-    (fp16 values), B_dev: K x N float16 device tensor.  Returns a list of
+    (fp16 values), B_dev: K x N device tensor (for transposed=True: the
+    M x N float32 gradient, and the product is A^T B).  Returns a list of
     dicts sorted by time (us per launch)."""
     import numpy as np
     import torch
@@ -65,14 +66,19 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda"):
         values_dev = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(device)
     N = B_dev.shape[1]
     cands = candidates or [(r, t) for r in RPW_CANDIDATES for t in TILE_CANDIDATES]
-    out = torch.empty((A.n_rows, N), dtype=torch.float32, device=device)
+    out = torch.empty((A.n_cols if transposed else A.n_rows, N), dtype=torch.float32, device=device)
     stream = torch.cuda.current_stream()
     res = []
     plans = {}
+    fwd = Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=16) if transposed else None
+    fwd_tiled = fwd.pack(values_dev) if transposed else None
     for rpw, tile in cands:
         if rpw not in plans:
-            p = Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
-            plans[rpw] = (p, p.pack(values_dev))
+            if transposed:
+                plans[rpw] = (fwd.transpose(rpw), fwd_tiled)
+            else:
+                p = Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
+                plans[rpw] = (p, p.pack(values_dev))
         p, tiled = plans[rpw]
         for _ in range(3):
             spmm(p, tiled, B_dev, out=out, tile_n=tile)
@@ -95,11 +101,64 @@ def record(A, N, results, op="spmm", path=None):
     tab = _load()
     d = A.nnz / float(A.n_rows * A.n_cols)
     best = results[0]
+    M, K = (A.n_cols, A.n_rows) if op == "spmm_t" else (A.n_rows, A.n_cols)
     tab["entries"] = [e for e in tab["entries"]
-                      if not (e.get("op", "spmm") == op and e["M"] == A.n_rows and e["K"] == A.n_cols
+                      if not (e.get("op", "spmm") == op and e["M"] == M and e["K"] == K
                               and e["N"] == N and abs(e["density"] - d) < 1e-3)]
-    tab["entries"].append({"op": op, "M": A.n_rows, "K": A.n_cols, "N": N, "density": round(d, 5),
+    tab["entries"].append({"op": op, "M": M, "K": K, "N": N, "density": round(d, 5),
                            "rows_per_wave": best["rows_per_wave"], "tile_n": best["tile_n"], "us": best["us"],
                            "candidates": results})
     with open(path or TABLE_PATH, "w") as f:
         json.dump(tab, f, indent=1)
+
+
+def main(argv=None):
+    """Tune the built-in workloads on the current device and write a table.
+
+        python -m tca_spmm.autotune --out tuning_table.json [--workloads plain_gpu,batch12,...]
+    """
+    import argparse
+
+    import numpy as np
+    import torch
+
+    from . import workloads
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--workloads", default="plain_gpu,batch12,precision,sparse_linear")
+    ap.add_argument("--iters", type=int, default=30)
+    args = ap.parse_args(argv)
+    global _table
+    _table = {"entries": []}
+    dev = torch.device("cuda:0")
+
+    def f16(bits):
+        return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
+
+    jobs = []
+    sel = set(args.workloads.split(","))
+    if "plain_gpu" in sel:
+        jobs += [(c.A, c.B, None) for c in workloads.plain_gpu_suite()]
+    if "batch12" in sel:
+        jobs += [(c.A, c.B, c.extra["G"]) for c in workloads.batch12()]
+    if "precision" in sel:
+        c = workloads.precision()
+        jobs.append((c.A, c.B, None))
+    if "sparse_linear" in sel:
+        c = workloads.sparse_linear()
+        jobs.append((c.A, c.B, c.extra["G"]))
+    for A, B, G in jobs:
+        r = tune(A, f16(B), iters=args.iters)
+        record(A, B.shape[1], r, op="spmm", path=args.out)
+        print(f"spmm   {A.name:40s} N={B.shape[1]:5d} best rpw={r[0]['rows_per_wave']} tile={r[0]['tile_n']} "
+              f"{r[0]['us']:.1f} us (worst {r[-1]['us']:.1f})", flush=True)
+        if G is not None:
+            rt = tune(A, torch.from_numpy(G).to(dev), iters=args.iters, transposed=True)
+            record(A, G.shape[1], rt, op="spmm_t", path=args.out)
+            print(f"spmm_t {A.name:40s} N={G.shape[1]:5d} best rpw={rt[0]['rows_per_wave']} tile={rt[0]['tile_n']} "
+                  f"{rt[0]['us']:.1f} us (worst {rt[-1]['us']:.1f})", flush=True)
+
+
+if __name__ == "__main__":
+    main()
