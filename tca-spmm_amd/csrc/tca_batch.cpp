@@ -39,6 +39,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         tca::BatchItemDev &d = dev[i];
         d = tca::BatchItemDev{};
         d.n_rows = t.n_rows;
+        d.nnz = t.nnz;
         d.N = it.N;
         d.rpw = t.rpw;
         d.grp_chunk_ptr = p->d_grp_chunk_ptr;

@@ -90,7 +90,7 @@ tca_status launch_warm_up(hipStream_t stream);
 // Batched persistent launch: one descriptor per item, one record per work unit.
 struct BatchItemDev {
     int32_t variant;   // kernel variant id (batch_variant)
-    int32_t n_rows, N, ldb, ldc, n_tiles, rpw, pad;
+    int32_t n_rows, N, ldb, ldc, n_tiles, rpw, nnz;
     const int32_t *grp_chunk_ptr, *chunk_cols, *seg_ptr;
     const uint16_t *entry_slot;
     const int32_t *entry_src;

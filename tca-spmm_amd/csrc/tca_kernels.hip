@@ -137,7 +137,7 @@ struct SpmmParams {
     const float *row_scale;
     const float *col_scale;
     const int4 *units;       // standalone launch: (group, piece, c_lo, c_hi) per workgroup row
-    int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw, split;
+    int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw, split, nnz;
 };
 
 template <int ES> struct Store;
@@ -171,13 +171,16 @@ __device__ inline typename Store<KPL * ES>::T pack_k(const Oct (&rows)[KPL], int
 // Entries prefetched per lane per panel and chunk (covers 128 entries of a
 // 16x32 panel tile; denser tiles take an extra, unprefetched pass).
 constexpr int kEP = 2;
+// Chunks whose gathered data is in flight while one is computed.
+constexpr int kDepth = 2;
 
 // One wave = one (group g, N-tile `tile`, chunks [c_lo, c_hi) of g).  RPW rows
 // (PPG = RPW/16 panels) x NT columns; lds_b holds NT*32 elements, lds_a
-// PPG*512 elements.  Software pipeline, one chunk per iteration:
-//   stage A (two chunks ahead): column ids and segment bounds,
-//   stage B (one chunk ahead):  gathered B rows and the panel entries,
-//   stage C (this chunk):       LDS transpose + scatter, MFMAs, un-scatter.
+// PPG*512 elements.  Software pipeline over a ring of kDepth register stages
+// (indexed by compile-time slot, so nothing is copied between iterations):
+//   index loads (column ids, segment bounds) run 2*kDepth chunks ahead,
+//   data loads (gathered B rows, panel entries) run kDepth chunks ahead,
+//   the current chunk is transposed / scattered into LDS and fed to the MFMAs.
 // atomic: several waves share the output tile (chunk ranges split a group);
 // the caller has zeroed C and the wave adds its partial sums.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
@@ -191,8 +194,21 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     constexpr int LPS = 64 / SEGS;     // lanes sharing one segment
     constexpr int KPL = 32 / LPS;      // consecutive k rows per lane (= NT/16)
     constexpr int OCT = 8 * ES;        // octet bytes
+    constexpr int D = kDepth;
     using Oct = typename Elt<AT>::Oct;
     using E = typename Store<ES>::T;
+
+    struct Index {
+        int cols[KPL];
+        int segb;                      // lane j <= PPG: seg_ptr[c*PPG + j]
+    };
+    struct Stage {
+        Oct rows[KPL];
+        uint32_t rows_ok;              // bit j: rows[j] is a real B row (else zeros)
+        int slot[PPG][kEP];            // -1: no entry
+        uint32_t val[PPG][kEP];        // value bits, or entry_src when INDIRECT
+        int e0[PPG], e1[PPG];
+    };
 
     const int lane = threadIdx.x;
     const int n0 = tile * NT;
@@ -209,98 +225,72 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     const bool col_ok = ncol < p.N;
     const E *avals = static_cast<const E *>(p.a);
     const int seg_lane = lane <= PPG ? lane : PPG;
+    const int last_entry = p.nnz > 0 ? p.nnz - 1 : 0;
 
 #pragma unroll
     for (int i = 0; i < PPG; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
 
-    // ---- pipeline registers
-    int cols_n[KPL];               // column ids of chunk c+1
-    int segb_n = 0;                // lane j <= PPG: seg_ptr[(c+1)*PPG + j]
-    Oct rows_c[KPL];               // B rows of chunk c
-    int ent_c[PPG][kEP];           // entry index of chunk c (-1: none)
-    int slot_c[PPG][kEP];
-    uint32_t val_c[PPG][kEP];      // value bits (non-indirect) / entry_src (indirect)
-    int e0_c[PPG], e1_c[PPG];
-
-    auto load_index = [&](int c, int (&cols)[KPL], int &segb) {
+    auto load_index = [&](int c, Index &ix) {
         const int32_t *cc = p.chunk_cols + (long)c * 32 + k0;
 #pragma unroll
-        for (int j = 0; j < KPL; j++) cols[j] = cc[j];
-        segb = p.seg_ptr[c * PPG + seg_lane];
+        for (int j = 0; j < KPL; j++) ix.cols[j] = cc[j];
+        ix.segb = p.seg_ptr[c * PPG + seg_lane];
     };
-    auto load_data = [&](const int (&cols)[KPL], int segb, Oct (&rows)[KPL], int (&ent)[PPG][kEP], int (&slot)[PPG][kEP],
-                         uint32_t (&val)[PPG][kEP], int (&e0)[PPG], int (&e1)[PPG]) {
+    // Loads are issued unconditionally (addresses clamped into range, results
+    // masked when used) so that the waits the compiler places stay counted
+    // per load instead of draining the whole queue at each branch.
+    auto load_data = [&](const Index &ix, Stage &st) {
+        st.rows_ok = 0;
 #pragma unroll
-        for (int j = 0; j < KPL; j++)
-            rows[j] = (cols[j] >= 0 && col_ok) ? RowLoad<AT, BT>::load(p.B, (long)cols[j] * p.ldb + ncol) : zero_oct<Oct>();
+        for (int j = 0; j < KPL; j++) {
+            const bool ok = ix.cols[j] >= 0 && col_ok;
+            st.rows_ok |= ok ? (1u << j) : 0u;
+            const long off = ok ? (long)ix.cols[j] * p.ldb + ncol : 0;
+            st.rows[j] = RowLoad<AT, BT>::load(p.B, off);
+        }
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
-            e0[i] = __shfl(segb, i);
-            e1[i] = __shfl(segb, i + 1);
+            st.e0[i] = __shfl(ix.segb, i);
+            st.e1[i] = __shfl(ix.segb, i + 1);
 #pragma unroll
             for (int k = 0; k < kEP; k++) {
-                const int e = e0[i] + k * 64 + lane;
-                const bool ok = e < e1[i];
-                ent[i][k] = ok ? e : -1;
-                slot[i][k] = ok ? (int)p.entry_slot[e] : 0;
-                if (INDIRECT) val[i][k] = ok ? (uint32_t)p.entry_src[e] : 0u;
-                else val[i][k] = ok ? (uint32_t)avals[e] : 0u;
+                const int e = st.e0[i] + k * 64 + lane;
+                const bool ok = e < st.e1[i];
+                const int ec = ok ? e : last_entry;
+                const int sl = (int)p.entry_slot[ec];
+                st.slot[i][k] = ok ? sl : -1;
+                st.val[i][k] = INDIRECT ? (uint32_t)p.entry_src[ec] : (uint32_t)avals[ec];
             }
         }
     };
-
-    if (c_lo < c_hi) {
-        int cols_c[KPL], segb_c;
-        load_index(c_lo, cols_c, segb_c);
-        load_data(cols_c, segb_c, rows_c, ent_c, slot_c, val_c, e0_c, e1_c);
-        if (c_lo + 1 < c_hi) load_index(c_lo + 1, cols_n, segb_n);
-    }
-
-    for (int c = c_lo; c < c_hi; ++c) {
-        // stage B for c+1 and stage A for c+2 (issued before this chunk's work)
-        Oct rows_n[KPL];
-        int ent_n[PPG][kEP], slot_n[PPG][kEP], e0_n[PPG], e1_n[PPG];
-        uint32_t val_n[PPG][kEP];
-        int cols_nn[KPL], segb_nn = 0;
-#pragma unroll
-        for (int j = 0; j < KPL; j++) cols_nn[j] = -1;
-        const bool has_n = c + 1 < c_hi;
-        if (has_n) {
-            load_data(cols_n, segb_n, rows_n, ent_n, slot_n, val_n, e0_n, e1_n);
-            if (c + 2 < c_hi) load_index(c + 2, cols_nn, segb_nn);
-        }
-        // values of this chunk (indirect: through the forward tiled order)
-        E v_c[PPG][kEP];
+    auto a_off = [&](int i, int s) { return (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES; };
+    auto compute = [&](const Stage &st) {
+        E v[PPG][kEP];
 #pragma unroll
         for (int i = 0; i < PPG; i++)
 #pragma unroll
-            for (int k = 0; k < kEP; k++)
-                v_c[i][k] = INDIRECT ? (ent_c[i][k] >= 0 ? avals[val_c[i][k]] : (E)0) : (E)val_c[i][k];
-
-        // stage C: transposed B tile, A scatter
+            for (int k = 0; k < kEP; k++) v[i][k] = INDIRECT ? avals[st.val[i][k]] : (E)st.val[i][k];
+        Oct rows[KPL];
+#pragma unroll
+        for (int j = 0; j < KPL; j++) rows[j] = (st.rows_ok >> j) & 1u ? st.rows[j] : zero_oct<Oct>();
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int n = 8 * seg + e;
             const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
-            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows_c, e);
+            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
         }
         bool overflow[PPG];
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
 #pragma unroll
             for (int k = 0; k < kEP; k++)
-                if (ent_c[i][k] >= 0) {
-                    const int s = slot_c[i][k];
-                    *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = v_c[i][k];
+                if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
+            overflow[i] = st.e1[i] - st.e0[i] > kEP * 64;
+            if (overflow[i])
+                for (int e = st.e0[i] + kEP * 64 + lane; e < st.e1[i]; e += 64) {
+                    const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
+                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_slot[e])) = vv;
                 }
-            overflow[i] = e1_c[i] - e0_c[i] > kEP * 64;
-            if (overflow[i]) {
-                for (int e = e0_c[i] + kEP * 64 + lane; e < e1_c[i]; e += 64) {
-                    const int s = p.entry_slot[e];
-                    const E v = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                    *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = v;
-                }
-            }
         }
         __syncthreads();
         Oct a[PPG];
@@ -321,24 +311,32 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             } else {
 #pragma unroll
                 for (int k = 0; k < kEP; k++)
-                    if (ent_c[i][k] >= 0) {
-                        const int s = slot_c[i][k];
-                        *reinterpret_cast<E *>(lds_a + (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES) = (E)0;
-                    }
+                    if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
             }
         }
-        // rotate the pipeline
-        if (has_n) {
+    };
+
+    // Prefetches past the last chunk re-read the last chunk (clamped index),
+    // so every load is unconditional; only the compute step is guarded.
+    if (c_lo < c_hi) {
+        const int c_last = c_hi - 1;
+        Index ix[D];
+        Stage st[D];
 #pragma unroll
-            for (int j = 0; j < KPL; j++) { rows_c[j] = rows_n[j]; cols_n[j] = cols_nn[j]; }
+        for (int s = 0; s < D; s++) load_index(min(c_lo + s, c_last), ix[s]);
 #pragma unroll
-            for (int i = 0; i < PPG; i++) {
-                e0_c[i] = e0_n[i];
-                e1_c[i] = e1_n[i];
+        for (int s = 0; s < D; s++) {
+            load_data(ix[s], st[s]);
+            load_index(min(c_lo + s + D, c_last), ix[s]);
+        }
+        for (int c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
-                for (int k = 0; k < kEP; k++) { ent_c[i][k] = ent_n[i][k]; slot_c[i][k] = slot_n[i][k]; val_c[i][k] = val_n[i][k]; }
+            for (int s = 0; s < D; s++) {
+                const int c = c0 + s;
+                if (c < c_hi) compute(st[s]);
+                load_data(ix[s], st[s]);
+                load_index(min(c + 2 * D, c_last), ix[s]);
             }
-            segb_n = segb_nn;
         }
     }
     __syncthreads();  // LDS is clean for the next unit
@@ -627,6 +625,7 @@ __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
     p.col_scale = nullptr;
     p.units = nullptr;
     p.split = 0;
+    p.nnz = d.nnz;
     p.n_rows = d.n_rows;
     p.N = d.N;
     p.ldb = d.ldb;
@@ -777,7 +776,7 @@ int default_tile_n(const tca_plan *p, int N)
 // split into pieces so that roughly kTargetWaves waves run; pieces keep at
 // least kMinPieceChunks chunks.  Returns the device unit table for the plan.
 static constexpr int kTargetWaves = 2048;
-static constexpr int kMinPieceChunks = 4;
+static constexpr int kMinPieceChunks = 12;
 
 static tca_status spmm_units(const tca_plan *cp, int n_tiles, const int4 **units, int *n_units, bool *split)
 {
@@ -846,6 +845,7 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.n_tiles = (N + tile_n - 1) / tile_n;
     prm.n_groups = t.n_groups;
     prm.rpw = t.rpw;
+    prm.nnz = t.nnz;
     int n_units = 0;
     bool split = false;
     tca_status st = spmm_units(p, prm.n_tiles, &prm.units, &n_units, &split);

@@ -99,6 +99,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libtca_spmm.so not found at {LIB_PATH}: build it first "
                               "(python -c 'import __graft_entry__ as g; g.build()' or make -C tca-spmm_amd)")
+        # One HIP runtime per process: torch bundles libamdhip64 (soname
+        # libamdhip64.so.7); loading torch first makes this library bind to that
+        # copy instead of pulling in a second runtime from /opt/rocm.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
