@@ -75,33 +75,47 @@ __device__ inline uint32_t oct_elem(const uint2 &o, int e)
 __device__ inline uint32_t f32_to_f16_bits(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f); }
 __device__ inline uint32_t f32_to_bf16_bits(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)f); }
 
-// Load 8 consecutive elements of dense row `row` (columns n..n+7) and convert
-// them to the MFMA element type AT.
+// 8 consecutive elements of a dense row, loaded raw in the storage type BT and
+// converted to the MFMA element type AT only when used, so that a prefetched
+// load is not waited for at the point where it is issued.
 template <tca_dtype AT, tca_dtype BT> struct RowLoad;
 template <tca_dtype AT> struct RowLoad<AT, AT> {
     using Oct = typename Elt<AT>::Oct;
-    __device__ static Oct load(const void *B, long off)
+    using Raw = Oct;
+    __device__ static Raw load(const void *B, long off)
     {
         using E = typename std::conditional<Elt<AT>::ES == 2, uint16_t, uint8_t>::type;
         return *reinterpret_cast<const Oct *>(static_cast<const E *>(B) + off);
     }
+    __device__ static Oct cvt(const Raw &r) { return r; }
+};
+struct F32x8 {
+    float4 a, b;
 };
 template <> struct RowLoad<TCA_F16, TCA_F32> {
-    __device__ static uint4 load(const void *B, long off)
+    using Raw = F32x8;
+    __device__ static Raw load(const void *B, long off)
     {
-        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
-        const float4 a = p[0], b = p[1];
-        return make_uint4(f32_to_f16_bits(a.x) | f32_to_f16_bits(a.y) << 16, f32_to_f16_bits(a.z) | f32_to_f16_bits(a.w) << 16,
-                          f32_to_f16_bits(b.x) | f32_to_f16_bits(b.y) << 16, f32_to_f16_bits(b.z) | f32_to_f16_bits(b.w) << 16);
+        const float4 *q = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
+        return F32x8{q[0], q[1]};
+    }
+    __device__ static uint4 cvt(const Raw &r)
+    {
+        return make_uint4(f32_to_f16_bits(r.a.x) | f32_to_f16_bits(r.a.y) << 16, f32_to_f16_bits(r.a.z) | f32_to_f16_bits(r.a.w) << 16,
+                          f32_to_f16_bits(r.b.x) | f32_to_f16_bits(r.b.y) << 16, f32_to_f16_bits(r.b.z) | f32_to_f16_bits(r.b.w) << 16);
     }
 };
 template <> struct RowLoad<TCA_BF16, TCA_F32> {
-    __device__ static uint4 load(const void *B, long off)
+    using Raw = F32x8;
+    __device__ static Raw load(const void *B, long off)
     {
-        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
-        const float4 a = p[0], b = p[1];
-        return make_uint4(f32_to_bf16_bits(a.x) | f32_to_bf16_bits(a.y) << 16, f32_to_bf16_bits(a.z) | f32_to_bf16_bits(a.w) << 16,
-                          f32_to_bf16_bits(b.x) | f32_to_bf16_bits(b.y) << 16, f32_to_bf16_bits(b.z) | f32_to_bf16_bits(b.w) << 16);
+        const float4 *q = reinterpret_cast<const float4 *>(static_cast<const float *>(B) + off);
+        return F32x8{q[0], q[1]};
+    }
+    __device__ static uint4 cvt(const Raw &r)
+    {
+        return make_uint4(f32_to_bf16_bits(r.a.x) | f32_to_bf16_bits(r.a.y) << 16, f32_to_bf16_bits(r.a.z) | f32_to_bf16_bits(r.a.w) << 16,
+                          f32_to_bf16_bits(r.b.x) | f32_to_bf16_bits(r.b.y) << 16, f32_to_bf16_bits(r.b.z) | f32_to_bf16_bits(r.b.w) << 16);
     }
 };
 
@@ -202,8 +216,9 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         int cols[KPL];
         int segb;                      // lane j <= PPG: seg_ptr[c*PPG + j]
     };
+    using RL = RowLoad<AT, BT>;
     struct Stage {
-        Oct rows[KPL];
+        typename RL::Raw rows[KPL];
         uint32_t rows_ok;              // bit j: rows[j] is a real B row (else zeros)
         int slot[PPG][kEP];            // -1: no entry
         uint32_t val[PPG][kEP];        // value bits, or entry_src when INDIRECT
@@ -246,7 +261,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             const bool ok = ix.cols[j] >= 0 && col_ok;
             st.rows_ok |= ok ? (1u << j) : 0u;
             const long off = ok ? (long)ix.cols[j] * p.ldb + ncol : 0;
-            st.rows[j] = RowLoad<AT, BT>::load(p.B, off);
+            st.rows[j] = RL::load(p.B, off);
         }
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
@@ -272,7 +287,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             for (int k = 0; k < kEP; k++) v[i][k] = INDIRECT ? avals[st.val[i][k]] : (E)st.val[i][k];
         Oct rows[KPL];
 #pragma unroll
-        for (int j = 0; j < KPL; j++) rows[j] = (st.rows_ok >> j) & 1u ? st.rows[j] : zero_oct<Oct>();
+        for (int j = 0; j < KPL; j++) rows[j] = (st.rows_ok >> j) & 1u ? RL::cvt(st.rows[j]) : zero_oct<Oct>();
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int n = 8 * seg + e;
@@ -433,12 +448,22 @@ struct SddmmParams {
     const int32_t *unit_chunk;   // per workgroup: first chunk
 };
 
-// One wave = (group g, chunks [c0, c0+CB) of that group); `tile` is 16*33 floats of LDS.
+// One wave = (group g, chunks [c0, c0+CB) of that group); `tile` is 16*33
+// floats of LDS.  The contraction over N runs as a two-stage register
+// pipeline: the G and X octets of step n+32 are loaded (unconditionally, with
+// row addresses clamped into range) while step n is multiplied.  Rows past the
+// matrix and padded columns only feed accumulator cells no entry samples.
 template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
 __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, float *tile)
 {
     constexpr int PPG = RPW / 16;
     using Oct = typename Elt<AT>::Oct;
+    using GR = RowLoad<AT, GT>;
+    using XR = RowLoad<AT, AT>;
+    struct Step {
+        typename GR::Raw g[PPG];
+        typename XR::Raw x[CB][2];
+    };
 
     const int lane = threadIdx.x;
     const int c1 = min(c0 + CB, p.grp_chunk_ptr[g + 1]);
@@ -452,33 +477,54 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
 #pragma unroll
             for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // X rows gathered by this lane: chunk b, half h -> column chunk_cols[(c0+b)*32 + 16h + lane%16]
-    int xcol[CB][2];
+    long xoff[CB][2];   // X row of chunk b, half h, this lane's column (row 0 when absent)
 #pragma unroll
     for (int b = 0; b < CB; b++)
 #pragma unroll
-        for (int h = 0; h < 2; h++) xcol[b][h] = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + 16 * h + (lane & 15)] : -1;
-    int grow[PPG];
+        for (int h = 0; h < 2; h++) {
+            const int col = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + 16 * h + (lane & 15)] : -1;
+            xoff[b][h] = (long)(col >= 0 ? col : 0) * p.ldx;
+        }
+    long goff[PPG];
 #pragma unroll
-    for (int i = 0; i < PPG; i++) grow[i] = g * RPW + 16 * i + (lane & 15);
-
+    for (int i = 0; i < PPG; i++) {
+        const int row = g * RPW + 16 * i + (lane & 15);
+        goff[i] = (long)(row < p.n_rows ? row : 0) * p.ldg;
+    }
     const int q8 = 8 * (lane >> 4);
-    for (int n = 0; n < p.N; n += 32) {
+    auto load = [&](int n, Step &st) {
+#pragma unroll
+        for (int i = 0; i < PPG; i++) st.g[i] = GR::load(p.G, goff[i] + n + q8);
+#pragma unroll
+        for (int b = 0; b < CB; b++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) st.x[b][h] = XR::load(p.X, xoff[b][h] + n + q8);
+    };
+    auto comp = [&](const Step &st) {
         Oct ga[PPG];
 #pragma unroll
-        for (int i = 0; i < PPG; i++)
-            ga[i] = grow[i] < p.n_rows ? RowLoad<AT, GT>::load(p.G, (long)grow[i] * p.ldg + n + q8) : zero_oct<Oct>();
+        for (int i = 0; i < PPG; i++) ga[i] = GR::cvt(st.g[i]);
 #pragma unroll
         for (int b = 0; b < CB; b++) {
-            if (b >= nc) break;
+            if (b < nc) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int col = xcol[b][h];
-                const Oct xb = col >= 0 ? RowLoad<AT, AT>::load(p.X, (long)col * p.ldx + n + q8) : zero_oct<Oct>();
+                for (int h = 0; h < 2; h++) {
+                    const Oct xb = XR::cvt(st.x[b][h]);
 #pragma unroll
-                for (int i = 0; i < PPG; i++) acc[b][h][i] = Elt<AT>::mma(ga[i], xb, acc[b][h][i]);
+                    for (int i = 0; i < PPG; i++) acc[b][h][i] = Elt<AT>::mma(ga[i], xb, acc[b][h][i]);
+                }
             }
         }
+    };
+    const int n_last = p.N - 32;
+    Step st[2];
+    load(0, st[0]);
+    load(min(32, n_last), st[1]);
+    for (int n = 0; n < p.N; n += 64) {
+        comp(st[0]);
+        load(min(n + 64, n_last), st[0]);
+        if (n + 32 < p.N) comp(st[1]);
+        load(min(n + 96, n_last), st[1]);
     }
     // sample: stage each (chunk, panel) 16x32 tile in LDS, then read entry slots
 #pragma unroll
@@ -776,7 +822,7 @@ int default_tile_n(const tca_plan *p, int N)
 // split into pieces so that roughly kTargetWaves waves run; pieces keep at
 // least kMinPieceChunks chunks.  Returns the device unit table for the plan.
 static constexpr int kTargetWaves = 2048;
-static constexpr int kMinPieceChunks = 12;
+static constexpr int kMinPieceChunks = 4;
 
 static tca_status spmm_units(const tca_plan *cp, int n_tiles, const int4 **units, int *n_units, bool *split)
 {
