@@ -58,7 +58,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
-                                                     " (batched SpMM: f16 A with f16 B forward, f32 B transposed)");
+                                                     " (batched SpMM: rows_per_wave 16/32, tile_n 16/32/64, f16 A with f16 B forward, f32 B transposed)");
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;

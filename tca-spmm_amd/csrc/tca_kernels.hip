@@ -138,6 +138,16 @@ __device__ inline int a_oct(int r, int q)
     return m * 8 + ((((r & 1) << 2) | q) ^ (m & 3));
 }
 
+// Orders one wave's LDS accesses (write -> read -> write of its own tiles).
+// LDS instructions of a wave execute in issue order, so only the compiler
+// has to be kept from moving memory operations across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct SpmmParams {
     const int32_t *grp_chunk_ptr;
     const int32_t *chunk_cols;
@@ -198,8 +208,8 @@ constexpr int kDepth = 2;
 // atomic: several waves share the output tile (chunk ranges split a group);
 // the caller has zeroed C and the wave adds its partial sums.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, bool atomic,
-                                          uint8_t *lds_b, uint8_t *lds_a)
+__device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, uint8_t *lds_b,
+                                          uint8_t *lds_a, f32x4 (&acc)[RPW / 16][NT / 16])
 {
     constexpr int PPG = RPW / 16;
     constexpr int ES = Elt<AT>::ES;
@@ -225,10 +235,10 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         int e0[PPG], e1[PPG];
     };
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int n0 = tile * NT;
+    (void)g;
 
-    f32x4 acc[PPG][NSUB];
 #pragma unroll
     for (int i = 0; i < PPG; i++)
 #pragma unroll
@@ -307,7 +317,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
                     *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_slot[e])) = vv;
                 }
         }
-        __syncthreads();
+        wave_lds_sync();
         Oct a[PPG];
 #pragma unroll
         for (int i = 0; i < PPG; i++) a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
@@ -317,7 +327,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
 #pragma unroll
             for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
         }
-        __syncthreads();
+        wave_lds_sync();
         // un-scatter: put the zeros back (a whole tile when it overflowed)
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
@@ -354,12 +364,20 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             }
         }
     }
-    __syncthreads();  // LDS is clean for the next unit
+    wave_lds_sync();  // this wave's LDS tiles are clean for its next use
+}
 
-    // epilogue: lane holds D[4*(lane/16)+j][lane%16] of each 16x16 tile
+// Direct FP32 stores (or atomic adds) of one wave's accumulators: lane holds
+// D[4*(lane/16)+j][lane%16] of each 16x16 tile.
+template <int RPW, int NT>
+__device__ __forceinline__ void spmm_store_direct(const SpmmParams &p, int g, int tile,
+                                                  const f32x4 (&acc)[RPW / 16][NT / 16], bool atomic)
+{
+    const int lane = threadIdx.x & 63;
+    const int n0 = tile * NT;
     const int rbase = g * RPW + 4 * (lane >> 4);
 #pragma unroll
-    for (int i = 0; i < PPG; i++) {
+    for (int i = 0; i < RPW / 16; i++) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int row = rbase + 16 * i + j;
@@ -367,7 +385,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
             float *crow = p.C + (long)row * p.ldc;
 #pragma unroll
-            for (int s = 0; s < NSUB; s++) {
+            for (int s = 0; s < NT / 16; s++) {
                 const int n = n0 + 16 * s + (lane & 15);
                 if (n < p.N) {
                     float v = acc[i][s][j] * rs;
@@ -380,16 +398,80 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     }
 }
 
-// Standalone launch: workgroup b runs unit b of the plan's unit table
-// (group, N-tile, chunk range), heavy groups first.
+// LDS bytes one wave of spmm_kernel needs: its B and A tiles during the chunk
+// loop, then its partial RPW x NT tile (rows padded by 4 floats).
+template <int RPW, int NT, tca_dtype AT>
+struct SpmmLds {
+    static constexpr int B = NT * 32 * Elt<AT>::ES;
+    static constexpr int A = (RPW / 16) * 512 * Elt<AT>::ES;
+    static constexpr int RED_LD = NT + 4;
+    static constexpr int RED = RPW * RED_LD * 4;
+    static constexpr int SLOT = (B + A) > RED ? (B + A) : RED;
+    static constexpr int WMAX = 65536 / SLOT >= 8 ? 8 : (65536 / SLOT >= 1 ? 65536 / SLOT : 1);
+};
+
+// Standalone launch: workgroup (x, y) = (unit x of the plan's unit table,
+// N-tile y).  Its W = blockDim.x/64 waves split the unit's chunk range
+// evenly, each into its own LDS slot; the W partial tiles are then summed in
+// wave order (deterministic) and stored as float4 rows.  Units of a group
+// split across workgroups (p.split) accumulate atomically into a zeroed C.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__global__ __launch_bounds__(64) void spmm_kernel(SpmmParams p)
+__global__ __launch_bounds__(64 * SpmmLds<RPW, NT, AT>::WMAX) void spmm_kernel(SpmmParams p)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[NT * 32 * Elt<AT>::ES];
-    __shared__ __attribute__((aligned(16))) uint8_t lds_a[(RPW / 16) * 512 * Elt<AT>::ES];
-    const int4 u = p.units[blockIdx.x];   // (group, split piece, c_lo, c_hi)
-    const int tile = blockIdx.y;
-    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, u.x, tile, u.z, u.w, p.split, lds_b, lds_a);
+    using L = SpmmLds<RPW, NT, AT>;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint8_t *mine = smem + w * L::SLOT;
+    const int4 u = p.units[blockIdx.x];   // (group, pieces, c_lo, c_hi)
+    const int g = u.x, tile = blockIdx.y;
+    const int nch = u.w - u.z;
+    const int lo = u.z + (nch * w) / W, hi = u.z + (nch * (w + 1)) / W;
+    f32x4 acc[RPW / 16][NT / 16];
+    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, g, tile, lo, hi, mine, mine + L::B, acc);
+
+    float *red = reinterpret_cast<float *>(mine);
+#pragma unroll
+    for (int i = 0; i < RPW / 16; i++)
+#pragma unroll
+        for (int s = 0; s < NT / 16; s++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) red[(16 * i + 4 * (lane >> 4) + j) * L::RED_LD + 16 * s + (lane & 15)] = acc[i][s][j];
+    __syncthreads();
+    const int n0 = tile * NT;
+    for (int idx = threadIdx.x * 4; idx < RPW * NT; idx += blockDim.x * 4) {
+        const int r = idx / NT, c = idx - r * NT;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = 0; q < W; q++) {
+            const float4 t = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(smem + q * L::SLOT) +
+                                                               r * L::RED_LD + c);
+            v.x += t.x;
+            v.y += t.y;
+            v.z += t.z;
+            v.w += t.w;
+        }
+        const int row = g * RPW + r, n = n0 + c;
+        if (row >= p.n_rows || n >= p.N) continue;
+        const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+        v.x *= rs;
+        v.y *= rs;
+        v.z *= rs;
+        v.w *= rs;
+        if (p.col_scale) {
+            v.x *= p.col_scale[n];
+            v.y *= p.col_scale[n + 1];
+            v.z *= p.col_scale[n + 2];
+            v.w *= p.col_scale[n + 3];
+        }
+        float *dst = p.C + (long)row * p.ldc + n;
+        if (p.split) {
+            unsafeAtomicAdd(dst, v.x);
+            unsafeAtomicAdd(dst + 1, v.y);
+            unsafeAtomicAdd(dst + 2, v.z);
+            unsafeAtomicAdd(dst + 3, v.w);
+        } else {
+            *reinterpret_cast<float4 *>(dst) = v;
+        }
+    }
 }
 
 using SpmmFn = void (*)(SpmmParams);
@@ -652,8 +734,9 @@ __global__ void warm_up_kernel() {}
 // sorted heaviest first on the host.  counters[0] hands out units,
 // counters[1] counts finished workgroups; the last one resets both, so the
 // next launch (stream-ordered) starts from zero without a memset.
-// Variant ids: 0..11 forward f16 x f16 (rpw index * 4 + tile index),
-// 12..23 transposed f16 x f32, 24..26 SDDMM G f32, 27..29 SDDMM G f16.
+// Variant ids (rows_per_wave 16/32, tile_n 16/32/64): 0..5 forward f16 x f16
+// (rpw index * 3 + tile index), 6..11 transposed f16 x f32, 12..13 SDDMM with
+// an f32 G, 14..15 SDDMM with an f16 G.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -705,20 +788,28 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
 
 #define TCA_FWD_CASE(id, R, T)                                                                          \
     case id:                                                                                            \
-        spmm_unit<R, T, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, \
-                                                 lds_b, lds_a);                                         \
+        {                                                                                               \
+            f32x4 acc[R / 16][T / 16];                                                                  \
+            const SpmmParams sp = to_spmm(d);                                                           \
+            spmm_unit<R, T, TCA_F16, TCA_F16, false>(sp, un.g, un.t, un.c_lo, un.c_hi, lds_b, lds_a, acc); \
+            spmm_store_direct<R, T>(sp, un.g, un.t, acc, un.atomic != 0);                               \
+        }                                                                                               \
         break;
 #define TCA_BWD_CASE(id, R, T)                                                                          \
     case id:                                                                                            \
-        spmm_unit<R, T, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, \
-                                                lds_b, lds_a);                                          \
+        {                                                                                               \
+            f32x4 acc[R / 16][T / 16];                                                                  \
+            const SpmmParams sp = to_spmm(d);                                                           \
+            spmm_unit<R, T, TCA_F16, TCA_F32, true>(sp, un.g, un.t, un.c_lo, un.c_hi, lds_b, lds_a, acc); \
+            spmm_store_direct<R, T>(sp, un.g, un.t, acc, un.atomic != 0);                               \
+        }                                                                                               \
         break;
 
 __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                    const BatchUnitDev *__restrict__ units, int n_units, int *counters)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[128 * 32 * 2];
-    __shared__ __attribute__((aligned(16))) uint8_t lds_a[4 * 512 * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[64 * 32 * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_a[2 * 512 * 2 > 16 * 33 * 4 ? 2 * 512 * 2 : 16 * 33 * 4];
     const int lane = threadIdx.x;
     for (;;) {
         int u = 0;
@@ -728,18 +819,14 @@ __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restric
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
-            TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64) TCA_FWD_CASE(3, 16, 128)
-            TCA_FWD_CASE(4, 32, 16) TCA_FWD_CASE(5, 32, 32) TCA_FWD_CASE(6, 32, 64) TCA_FWD_CASE(7, 32, 128)
-            TCA_FWD_CASE(8, 64, 16) TCA_FWD_CASE(9, 64, 32) TCA_FWD_CASE(10, 64, 64) TCA_FWD_CASE(11, 64, 128)
-            TCA_BWD_CASE(12, 16, 16) TCA_BWD_CASE(13, 16, 32) TCA_BWD_CASE(14, 16, 64) TCA_BWD_CASE(15, 16, 128)
-            TCA_BWD_CASE(16, 32, 16) TCA_BWD_CASE(17, 32, 32) TCA_BWD_CASE(18, 32, 64) TCA_BWD_CASE(19, 32, 128)
-            TCA_BWD_CASE(20, 64, 16) TCA_BWD_CASE(21, 64, 32) TCA_BWD_CASE(22, 64, 64) TCA_BWD_CASE(23, 64, 128)
-        case 24: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 25: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 26: sddmm_unit<64, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 27: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 28: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 29: sddmm_unit<64, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+            TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64)
+            TCA_FWD_CASE(3, 32, 16) TCA_FWD_CASE(4, 32, 32) TCA_FWD_CASE(5, 32, 64)
+            TCA_BWD_CASE(6, 16, 16) TCA_BWD_CASE(7, 16, 32) TCA_BWD_CASE(8, 16, 64)
+            TCA_BWD_CASE(9, 32, 16) TCA_BWD_CASE(10, 32, 32) TCA_BWD_CASE(11, 32, 64)
+        case 12: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 13: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 14: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 15: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next unit
@@ -768,18 +855,18 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
 {
-    const int ri = rpw == 16 ? 0 : rpw == 32 ? 1 : rpw == 64 ? 2 : -1;
-    const int ti = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : tile_n == 128 ? 3 : -1;
+    const int ri = rpw == 16 ? 0 : rpw == 32 ? 1 : -1;
+    const int ti = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
     if (ri < 0) return -1;
     if (op == TCA_OP_SPMM) {
         if (ti < 0 || a_dtype != TCA_F16) return -1;
-        if (!transposed && b_dtype == TCA_F16) return ri * 4 + ti;
-        if (transposed && b_dtype == TCA_F32) return 12 + ri * 4 + ti;
+        if (!transposed && b_dtype == TCA_F16) return ri * 3 + ti;
+        if (transposed && b_dtype == TCA_F32) return 6 + ri * 3 + ti;
         return -1;
     }
     if (op == TCA_OP_SDDMM && b_dtype == TCA_F16 && !transposed) {
-        if (a_dtype == TCA_F32) return 24 + ri;
-        if (a_dtype == TCA_F16) return 27 + ri;
+        if (a_dtype == TCA_F32) return 12 + ri;
+        if (a_dtype == TCA_F16) return 14 + ri;
     }
     return -1;
 }
@@ -818,19 +905,25 @@ int default_tile_n(const tca_plan *p, int N)
     return 16;
 }
 
-// Chunk ranges per group: with few (group, N-tile) waves a group's chunks are
-// split into pieces so that roughly kTargetWaves waves run; pieces keep at
-// least kMinPieceChunks chunks.  Returns the device unit table for the plan.
-static constexpr int kTargetWaves = 2048;
-static constexpr int kMinPieceChunks = 4;
+// Work decomposition of a standalone SpMM launch.  A workgroup runs one unit
+// (group, chunk range) for one N-tile with W waves splitting the range.  W
+// grows (up to what 64 KB of LDS allows, and while each wave keeps >= 4 chunks)
+// until ~kTargetWaves waves exist; groups still longer than kMaxWaveChunks
+// chunks per wave are split into S units that accumulate atomically.
+static constexpr int kTargetWaves = 4096;
+static constexpr int kMaxWaveChunks = 32;
 
-static tca_status spmm_units(const tca_plan *cp, int n_tiles, const int4 **units, int *n_units, bool *split)
+static int spmm_slot_bytes(int rpw, int nt, int es)
+{
+    const int tiles = nt * 32 * es + (rpw / 16) * 512 * es;
+    const int red = rpw * (nt + 4) * 4;
+    return tiles > red ? tiles : red;
+}
+
+static tca_status spmm_units(const tca_plan *cp, int S, const int4 **units, int *n_units, bool *split)
 {
     tca_plan *p = const_cast<tca_plan *>(cp);
     const PlanTables &t = p->host;
-    const long waves = (long)t.n_groups * n_tiles;
-    int S = 1;
-    if (waves > 0 && waves < kTargetWaves / 2) S = (int)std::min<long>(16, (kTargetWaves + waves - 1) / waves);
     for (auto &u : p->unit_tables)
         if (u.pieces == S) {
             *units = static_cast<const int4 *>(u.dev);
@@ -843,7 +936,7 @@ static tca_status spmm_units(const tca_plan *cp, int n_tiles, const int4 **units
     for (int gi = 0; gi < t.n_groups; gi++) {
         const int g = t.work_order[gi];
         const int cb = t.grp_chunk_ptr[g], nch = t.grp_chunk_ptr[g + 1] - cb;
-        const int P = std::max(1, std::min(S, nch / kMinPieceChunks));
+        const int P = std::max(1, std::min(S, nch / 4));
         if (P > 1) any_split = true;
         for (int k = 0; k < P; k++)
             h.push_back(make_int4(g, P, cb + (int)((long)nch * k / P), cb + (int)((long)nch * (k + 1) / P)));
@@ -892,28 +985,32 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.n_groups = t.n_groups;
     prm.rpw = t.rpw;
     prm.nnz = t.nnz;
+    if (prm.n_tiles > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: too many N tiles");
+
+    const int es = a_dtype == TCA_E4M3 ? 1 : 2;
+    const int slot = spmm_slot_bytes(t.rpw, tile_n, es);
+    const int wcap = std::max(1, std::min(8, 65536 / slot));   // = SpmmLds<...>::WMAX
+    const long base = (long)t.n_groups * prm.n_tiles;
+    const int maxc = t.max_group_chunks;
+    int W = 1;
+    while (W * 2 <= wcap && maxc / (W * 2) >= 4 && base * W < kTargetWaves) W *= 2;
+    const int S = maxc > W * kMaxWaveChunks ? (maxc + W * kMaxWaveChunks - 1) / (W * kMaxWaveChunks) : 1;
+
     int n_units = 0;
     bool split = false;
-    tca_status st = spmm_units(p, prm.n_tiles, &prm.units, &n_units, &split);
+    tca_status st = spmm_units(p, S, &prm.units, &n_units, &split);
     if (st != TCA_OK) return st;
     prm.split = split ? 1 : 0;
-    if (split) {  // pieces accumulate: start from zero (a kernel, so graph capture stays memset-free)
+    if (split || n_units <= 0) {  // accumulate from zero (a kernel: no memset node under graph capture)
         const long total = (long)t.n_rows * N;
         const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
-        hipLaunchKernelGGL(zero_rows_kernel, dim3(blocks), dim3(256), 0, stream, C, t.n_rows, N, ldc);
-        TCA_HIP_TRY(hipGetLastError());
-    }
-    if (n_units <= 0) {
-        if (!split && t.n_rows > 0) {  // no nonzeros at all: C = 0
-            const long total = (long)t.n_rows * N;
-            const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+        if (blocks > 0) {
             hipLaunchKernelGGL(zero_rows_kernel, dim3(blocks), dim3(256), 0, stream, C, t.n_rows, N, ldc);
             TCA_HIP_TRY(hipGetLastError());
         }
-        return TCA_OK;
     }
-    if (prm.n_tiles > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: too many N tiles");
-    hipLaunchKernelGGL(fn, dim3((unsigned)n_units, (unsigned)prm.n_tiles), dim3(64), 0, stream, prm);
+    if (n_units <= 0) return TCA_OK;
+    hipLaunchKernelGGL(fn, dim3((unsigned)n_units, (unsigned)prm.n_tiles), dim3(64 * W), (size_t)W * slot, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
