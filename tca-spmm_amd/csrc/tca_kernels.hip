@@ -416,7 +416,7 @@ struct SpmmLds {
 // wave order (deterministic) and stored as float4 rows.  Units of a group
 // split across workgroups (p.split) accumulate atomically into a zeroed C.
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__global__ __launch_bounds__(64 * SpmmLds<RPW, NT, AT>::WMAX) void spmm_kernel(SpmmParams p)
+__global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel(SpmmParams p)
 {
     using L = SpmmLds<RPW, NT, AT>;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
