@@ -92,18 +92,19 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     layers, items, flops, nbytes = [], [], 0, 0
     for case in workloads.batch12(seed=0):
         W, X, G = case.A, case.B, case.extra["G"]
-        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N)
+        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True)
+        rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True)
         plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-        plan_t = plan.transpose()
+        plan_t = plan.transpose(rpw_t)
         w = plan.pack(torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev))
         Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
         Gd = torch.from_numpy(G).to(dev)
-        L = dict(plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile,
+        L = dict(plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile, tile_t=tile_t,
                  Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
                  dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
                  dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
         layers.append(L)
-        items += [T.spmm_item(plan, w, Xd, L["Y"], tile), T.spmm_item(plan_t, w, Gd, L["dX"], tile),
+        items += [T.spmm_item(plan, w, Xd, L["Y"], tile), T.spmm_item(plan_t, w, Gd, L["dX"], tile_t),
                   T.sddmm_item(plan, Gd, Xd, L["dW"])]
         flops += 3 * 2 * W.nnz * case.N
         nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
@@ -112,7 +113,7 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     def per_call():
         for L in layers:
             T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
-            T.spmm(L["plan_t"], L["w"], L["G"], out=L["dX"], tile_n=L["tile"])
+            T.spmm(L["plan_t"], L["w"], L["G"], out=L["dX"], tile_n=L["tile_t"])
             T.sddmm(L["plan"], L["G"], L["X"], out=L["dW"])
     bt = T.Batch(items)
     plain_eager = time_eager(torch, per_call, 20)

@@ -37,8 +37,15 @@ def heuristic(M, K, nnz, N):
     return rpw, tile
 
 
-def choice_for(M, K, nnz, N, op="spmm"):
-    """(rows_per_wave, tile_n) for an M x K matrix with nnz nonzeros and N columns."""
+BATCH_RPW = (16, 32)          # variants compiled into the persistent batch kernel
+BATCH_TILES = (16, 32, 64)
+
+
+def choice_for(M, K, nnz, N, op="spmm", batch=False):
+    """(rows_per_wave, tile_n) for an M x K matrix (op "spmm") or for the
+    product with the transpose of a K x M matrix whose transpose is M x K
+    (op "spmm_t"), with nnz nonzeros and N dense columns.  batch=True keeps
+    to the variants the persistent batch kernel has."""
     d = nnz / float(max(1, M * K))
     best, best_dist = None, None
     for e in _load()["entries"]:
@@ -48,8 +55,13 @@ def choice_for(M, K, nnz, N, op="spmm"):
         if best is None or dist < best_dist:
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
-        return best["rows_per_wave"], best["tile_n"]
-    return heuristic(M, K, nnz, N)
+        for c in best.get("candidates", [best]):
+            if not batch or (c["rows_per_wave"] in BATCH_RPW and c["tile_n"] in BATCH_TILES):
+                return c["rows_per_wave"], c["tile_n"]
+    rpw, tile = heuristic(M, K, nnz, N)
+    if batch:
+        rpw, tile = min(rpw, 32), min(tile, 64)
+    return rpw, tile
 
 
 def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False):

@@ -20,12 +20,13 @@ def _layers(cuda):
     out = []
     for case in workloads.batch12(seed=0):
         W, X, G = case.A, case.B, case.extra["G"]
-        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N)
+        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True)
+        rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True)
         plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-        plan_t = plan.transpose()
+        plan_t = plan.transpose(rpw_t)
         w = plan.pack(dev_f16(W.values))
         Xd, Gd = dev_f16(X), dev(G)
-        out.append(dict(case=case, plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile,
+        out.append(dict(case=case, plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile, tile_t=tile_t,
                         Y=torch.zeros((W.n_rows, case.N), dtype=torch.float32, device=cuda),
                         dX=torch.zeros((W.n_cols, case.N), dtype=torch.float32, device=cuda),
                         dW=torch.zeros(W.nnz, dtype=torch.float32, device=cuda)))
@@ -36,7 +37,7 @@ def _items(layers):
     items = []
     for L in layers:
         items.append(T.spmm_item(L["plan"], L["w"], L["X"], L["Y"], L["tile"]))
-        items.append(T.spmm_item(L["plan_t"], L["w"], L["G"], L["dX"], L["tile"]))
+        items.append(T.spmm_item(L["plan_t"], L["w"], L["G"], L["dX"], L["tile_t"]))
         items.append(T.sddmm_item(L["plan"], L["G"], L["X"], L["dW"]))
     return items
 
@@ -49,7 +50,7 @@ def test_batch_equals_per_call_and_oracle(cuda):
     torch.cuda.synchronize()
     for L in layers:
         Y = T.spmm(L["plan"], L["w"], L["X"], tile_n=L["tile"])
-        dX = T.spmm(L["plan_t"], L["w"], L["G"], tile_n=L["tile"])
+        dX = T.spmm(L["plan_t"], L["w"], L["G"], tile_n=L["tile_t"])
         dW = T.sddmm(L["plan"], L["G"], L["X"])
         assert torch.equal(dW, L["dW"])
         for got, ref in ((L["Y"], Y), (L["dX"], dX)):
