@@ -92,20 +92,32 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     layers, items, flops, nbytes = [], [], 0, 0
     for case in workloads.batch12(seed=0):
         W, X, G = case.A, case.B, case.extra["G"]
-        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True)
-        rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True)
-        plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-        plan_t = plan.transpose(rpw_t)
-        w = plan.pack(torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev))
+        w_vals = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
         Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
         Gd = torch.from_numpy(G).to(dev)
-        L = dict(plan=plan, plan_t=plan_t, w=w, X=Xd, G=Gd, tile=tile, tile_t=tile_t,
+        L = dict(X=Xd, G=Gd,
                  Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
                  dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
                  dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
+        # per-call path: the tuned best launch for each product
+        rpw, L["tile"] = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N)
+        rpw_t, L["tile_t"] = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t")
+        L["plan"] = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+        L["w"] = L["plan"].pack(w_vals)
+        fwd16 = L["plan"] if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+        L["plan_t"] = fwd16.transpose(rpw_t)
+        L["w16"] = L["w"] if rpw == 16 else fwd16.pack(w_vals)
+        # batched path: the best of the variants the persistent kernel has
+        rpw_b, tile_b = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True)
+        rpw_bt, tile_bt = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True)
+        pb = fwd16 if rpw_b == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw_b)
+        wb = L["w16"] if rpw_b == 16 else pb.pack(w_vals)
+        pbt = L["plan_t"] if rpw_bt == rpw_t else fwd16.transpose(rpw_bt)
+        L["keep"] = (pb, wb, pbt)
         layers.append(L)
-        items += [T.spmm_item(plan, w, Xd, L["Y"], tile), T.spmm_item(plan_t, w, Gd, L["dX"], tile_t),
-                  T.sddmm_item(plan, Gd, Xd, L["dW"])]
+        items += [T.spmm_item(pb, wb, Xd, L["Y"], tile_b), T.spmm_item(pbt, L["w16"], Gd, L["dX"], tile_bt),
+                  T.sddmm_item(fwd16, Gd, Xd, L["dW"])]
+        L["sddmm_plan"] = fwd16
         flops += 3 * 2 * W.nnz * case.N
         nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
                    + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, case.N))
@@ -113,8 +125,8 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     def per_call():
         for L in layers:
             T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
-            T.spmm(L["plan_t"], L["w"], L["G"], out=L["dX"], tile_n=L["tile_t"])
-            T.sddmm(L["plan"], L["G"], L["X"], out=L["dW"])
+            T.spmm(L["plan_t"], L["w16"], L["G"], out=L["dX"], tile_n=L["tile_t"])
+            T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
     bt = T.Batch(items)
     plain_eager = time_eager(torch, per_call, 20)
     plain_graph = time_graph(torch, per_call, 20)

@@ -58,7 +58,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
-                                                     " (batched SpMM: rows_per_wave 16/32, tile_n 16/32/64, f16 A with f16 B forward, f32 B transposed)");
+                                                     " (batched SpMM: rows_per_wave 16; f16 A with f16 B forward, tile_n 16/32/64; "
+                                                     "f16 A with f32 B transposed, tile_n 16/32)");
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;
@@ -76,7 +77,9 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 !aligned16(it.a) || !aligned16(it.b) || (t.nnz > 0 && !it.out))
                 return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM item " + std::to_string(i) + " has bad shapes/buffers");
             d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false);
-            if (d.variant < 0) return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i));
+            if (d.variant < 0)
+                return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) +
+                                                     " (batched SDDMM: rows_per_wave 16, f16 X, f32 or f16 G)");
             d.ldb = it.ldb;   // X
             d.ldc = it.lda;   // G
             for (int gi = 0; gi < t.n_groups; gi++) {
@@ -96,7 +99,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     int dev_id = 0, cus = 0;
     TCA_HIP_TRY(hipGetDevice(&dev_id));
     TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
-    const int workers = std::max(cus, 1) * 8;
+    const int workers = std::max(cus, 1) * tca::batch_blocks_per_cu();
     double total_cost = 0.0;
     for (double c : cost) total_cost += c;
     const double share = total_cost / workers;

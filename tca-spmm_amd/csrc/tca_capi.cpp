@@ -253,5 +253,6 @@ extern "C" tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, 
 {
     if (K < 0 || N < 0 || ldb < N || ldq < N || (K > 0 && N > 0 && (!d_B || !d_q || !d_col_scale)))
         return fail(TCA_ERR_INVALID_ARG, "tca_quantize_cols_e4m3: bad argument");
+    if ((K + 63) / 64 > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_quantize_cols_e4m3: K above 4194240");
     return tca::launch_quantize_cols(K, N, d_B, ldb, d_q, ldq, d_col_scale, static_cast<hipStream_t>(stream));
 }

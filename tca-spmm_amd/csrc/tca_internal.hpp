@@ -109,6 +109,8 @@ struct ZeroDesc {              // an output that split units accumulate into
 constexpr int kBatchSddmmCB = 4;
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed);
+// resident one-wave workgroups of batch_kernel per compute unit
+int batch_blocks_per_cu();
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
                         const ZeroDesc *zero, int n_zero, hipStream_t stream);
 

@@ -148,6 +148,16 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane index that the compiler cannot treat as loop-invariant: inside the
+// persistent batch loop, values derived from the lane would otherwise be
+// hoisted out of the loop for every variant and stay live across all of them.
+__device__ __forceinline__ int lane_id()
+{
+    int l = (int)threadIdx.x & 63;
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 struct SpmmParams {
     const int32_t *grp_chunk_ptr;
     const int32_t *chunk_cols;
@@ -160,9 +170,39 @@ struct SpmmParams {
     float *C;
     const float *row_scale;
     const float *col_scale;
-    const int4 *units;       // standalone launch: (group, piece, c_lo, c_hi) per workgroup row
+    const int4 *units;       // standalone launch: (group, piece, c_lo, c_hi) per work unit
     int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw, split, nnz;
+    int n_units;             // standalone launch: entries of `units`
+    int xcd_share;           // standalone launch: XCDs sharing one N-tile (see xcd_work)
 };
+
+// Workgroup -> (unit, N-tile) for a 1-D standalone grid.  Workgroups are
+// dispatched round-robin over the 8 XCDs (each with its own L2), so
+// workgroup L runs on XCD L % 8.  Every workgroup of one N-tile reads the
+// same gathered B columns, so the tiles are dealt to XCDs:
+//   xcd_share == 0: n_tiles % 8 == 0, XCD x runs tiles x, x+8, ... (all units);
+//   xcd_share == k (1..8, k * n_tiles == 8): k XCDs share a tile, taking its
+//     units round-robin so that heavy and light units stay spread;
+//   xcd_share < 0: plain order (tile = L / n_units).
+// Returns false for the padding workgroups of a grid rounded up to 8.
+__device__ __forceinline__ bool xcd_work(const SpmmParams &p, int L, int &unit, int &tile)
+{
+    if (p.xcd_share < 0) {
+        tile = L / p.n_units;
+        unit = L - tile * p.n_units;
+        return true;
+    }
+    const int x = L & 7, l = L >> 3;
+    if (p.xcd_share == 0) {
+        const int r = l / p.n_units;
+        unit = l - r * p.n_units;
+        tile = x + 8 * r;
+    } else {
+        tile = x % p.n_tiles;
+        unit = l * p.xcd_share + x / p.n_tiles;
+    }
+    return unit < p.n_units && tile < p.n_tiles;
+}
 
 template <int ES> struct Store;
 template <> struct Store<1> { using T = uint8_t; };
@@ -235,7 +275,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         int e0[PPG], e1[PPG];
     };
 
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_id();
     const int n0 = tile * NT;
     (void)g;
 
@@ -373,7 +413,7 @@ template <int RPW, int NT>
 __device__ __forceinline__ void spmm_store_direct(const SpmmParams &p, int g, int tile,
                                                   const f32x4 (&acc)[RPW / 16][NT / 16], bool atomic)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_id();
     const int n0 = tile * NT;
     const int rbase = g * RPW + 4 * (lane >> 4);
 #pragma unroll
@@ -410,8 +450,8 @@ struct SpmmLds {
     static constexpr int WMAX = 65536 / SLOT >= 8 ? 8 : (65536 / SLOT >= 1 ? 65536 / SLOT : 1);
 };
 
-// Standalone launch: workgroup (x, y) = (unit x of the plan's unit table,
-// N-tile y).  Its W = blockDim.x/64 waves split the unit's chunk range
+// Standalone launch: workgroup -> (unit of the plan's unit table, N-tile) by
+// xcd_work.  Its W = blockDim.x/64 waves split the unit's chunk range
 // evenly, each into its own LDS slot; the W partial tiles are then summed in
 // wave order (deterministic) and stored as float4 rows.  Units of a group
 // split across workgroups (p.split) accumulate atomically into a zeroed C.
@@ -422,8 +462,10 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint8_t *mine = smem + w * L::SLOT;
-    const int4 u = p.units[blockIdx.x];   // (group, pieces, c_lo, c_hi)
-    const int g = u.x, tile = blockIdx.y;
+    int ui, tile;
+    if (!xcd_work(p, blockIdx.x, ui, tile)) return;
+    const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
+    const int g = u.x;
     const int nch = u.w - u.z;
     const int lo = u.z + (nch * w) / W, hi = u.z + (nch * (w + 1)) / W;
     f32x4 acc[RPW / 16][NT / 16];
@@ -547,7 +589,7 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
         typename XR::Raw x[CB][2];
     };
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int c1 = min(c0 + CB, p.grp_chunk_ptr[g + 1]);
     const int nc = c1 - c0;
 
@@ -705,24 +747,46 @@ __global__ __launch_bounds__(256) void quantize_rows_kernel(int n_rows, const in
     for (int e = e0 + lane; e < e1; e += 64) q[e] = f32_to_e4m3(f16_bits_to_f32(vals[e]) / s);
 }
 
-// 256 threads per 64 columns: 4 row phases reduce the column amax in LDS
-__global__ __launch_bounds__(256) void quantize_cols_kernel(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq,
-                                                            float *scale)
+// Column scales in three steps so that the K rows are spread over many
+// workgroups: (1) partial column amax of a row slice, merged with an atomic
+// max on the float bits (amax >= 0, so unsigned order is float order) into
+// `scale`, which a fill kernel has zeroed; (2) q = e4m3(v / (amax / 448));
+// (3) scale = amax / 448 (1 for an all-zero column).
+constexpr int kQuantRowsPerSlice = 64;
+
+__global__ __launch_bounds__(256) void quantize_cols_amax_kernel(int K, int N, const uint16_t *B, int ldb, float *amax_out)
 {
     __shared__ float part[4][64];
     const int cl = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int n = blockIdx.x * 64 + cl;
+    const int k0 = blockIdx.y * kQuantRowsPerSlice, k1 = min(K, k0 + kQuantRowsPerSlice);
     float amax = 0.0f;
     if (n < N)
-        for (int k = ph; k < K; k += 4) amax = fmaxf(amax, fabsf(f16_bits_to_f32(B[(long)k * ldb + n])));
+        for (int k = k0 + ph; k < k1; k += 4) amax = fmaxf(amax, fabsf(f16_bits_to_f32(B[(long)k * ldb + n])));
     part[ph][cl] = amax;
     __syncthreads();
-    const float m = fmaxf(fmaxf(part[0][cl], part[1][cl]), fmaxf(part[2][cl], part[3][cl]));
-    const float s = m > 0.0f ? m / 448.0f : 1.0f;
-    if (n < N) {
-        if (ph == 0) scale[n] = s;
-        for (int k = ph; k < K; k += 4) q[(long)k * ldq + n] = f32_to_e4m3(f16_bits_to_f32(B[(long)k * ldb + n]) / s);
+    if (ph == 0 && n < N) {
+        const float m = fmaxf(fmaxf(part[0][cl], part[1][cl]), fmaxf(part[2][cl], part[3][cl]));
+        atomicMax(reinterpret_cast<unsigned int *>(amax_out + n), __builtin_bit_cast(unsigned int, m));
     }
+}
+
+__device__ inline float col_scale_of(float amax) { return amax > 0.0f ? amax / 448.0f : 1.0f; }
+
+__global__ __launch_bounds__(256) void quantize_cols_apply_kernel(int K, int N, const uint16_t *B, int ldb, uint8_t *q,
+                                                                  int ldq, const float *amax)
+{
+    const long total = (long)K * N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i / N), n = (int)(i - (long)k * N);
+        q[(long)k * ldq + n] = f32_to_e4m3(f16_bits_to_f32(B[(long)k * ldb + n]) / col_scale_of(amax[n]));
+    }
+}
+
+__global__ __launch_bounds__(256) void quantize_cols_finalize_kernel(int N, float *scale)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) scale[n] = col_scale_of(scale[n]);
 }
 
 __global__ void warm_up_kernel() {}
@@ -734,9 +798,11 @@ __global__ void warm_up_kernel() {}
 // sorted heaviest first on the host.  counters[0] hands out units,
 // counters[1] counts finished workgroups; the last one resets both, so the
 // next launch (stream-ordered) starts from zero without a memset.
-// Variant ids (rows_per_wave 16/32, tile_n 16/32/64): 0..5 forward f16 x f16
-// (rpw index * 3 + tile index), 6..11 transposed f16 x f32, 12..13 SDDMM with
-// an f32 G, 14..15 SDDMM with an f16 G.
+// Variants (all rows_per_wave 16, so that the kernel's register budget -- the
+// maximum over its cases -- stays at the level of one 16-row standalone
+// variant and several waves fit on each SIMD): 0..2 forward f16 x f16 with
+// tile_n 16/32/64, 3..4 transposed f16 x f32 with tile_n 16/32, 5 SDDMM with
+// an f32 G, 6 SDDMM with an f16 G.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -762,6 +828,8 @@ __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
     p.n_tiles = d.n_tiles;
     p.n_groups = 0;
     p.rpw = d.rpw;
+    p.n_units = 0;
+    p.xcd_share = -1;
     return p;
 }
 
@@ -809,24 +877,20 @@ __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restric
                                                    const BatchUnitDev *__restrict__ units, int n_units, int *counters)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds_b[64 * 32 * 2];
-    __shared__ __attribute__((aligned(16))) uint8_t lds_a[2 * 512 * 2 > 16 * 33 * 4 ? 2 * 512 * 2 : 16 * 33 * 4];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_a[512 * 2 > 16 * 33 * 4 ? 512 * 2 : 16 * 33 * 4];
     const int lane = threadIdx.x;
     for (;;) {
         int u = 0;
         if (lane == 0) u = atomicAdd(&counters[0], 1);
-        u = __shfl(u, 0);
+        u = __builtin_amdgcn_readfirstlane(u);   // uniform: unit and item descriptors load into SGPRs
         if (u >= n_units) break;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
             TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64)
-            TCA_FWD_CASE(3, 32, 16) TCA_FWD_CASE(4, 32, 32) TCA_FWD_CASE(5, 32, 64)
-            TCA_BWD_CASE(6, 16, 16) TCA_BWD_CASE(7, 16, 32) TCA_BWD_CASE(8, 16, 64)
-            TCA_BWD_CASE(9, 32, 16) TCA_BWD_CASE(10, 32, 32) TCA_BWD_CASE(11, 32, 64)
-        case 12: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 13: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 14: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 15: sddmm_unit<32, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+            TCA_BWD_CASE(3, 16, 16) TCA_BWD_CASE(4, 16, 32)
+        case 5: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 6: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next unit
@@ -855,20 +919,25 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
 {
-    const int ri = rpw == 16 ? 0 : rpw == 32 ? 1 : -1;
-    const int ti = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
-    if (ri < 0) return -1;
+    if (rpw != 16) return -1;
     if (op == TCA_OP_SPMM) {
-        if (ti < 0 || a_dtype != TCA_F16) return -1;
-        if (!transposed && b_dtype == TCA_F16) return ri * 3 + ti;
-        if (transposed && b_dtype == TCA_F32) return 6 + ri * 3 + ti;
+        if (a_dtype != TCA_F16) return -1;
+        if (!transposed && b_dtype == TCA_F16) return tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
+        if (transposed && b_dtype == TCA_F32) return tile_n == 16 ? 3 : tile_n == 32 ? 4 : -1;
         return -1;
     }
     if (op == TCA_OP_SDDMM && b_dtype == TCA_F16 && !transposed) {
-        if (a_dtype == TCA_F32) return 12 + ri;
-        if (a_dtype == TCA_F16) return 14 + ri;
+        if (a_dtype == TCA_F32) return 5;
+        if (a_dtype == TCA_F16) return 6;
     }
     return -1;
+}
+
+int batch_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 64, 0) != hipSuccess || n < 1) n = 1;
+    return n;
 }
 
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
@@ -1010,7 +1079,20 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
         }
     }
     if (n_units <= 0) return TCA_OK;
-    hipLaunchKernelGGL(fn, dim3((unsigned)n_units, (unsigned)prm.n_tiles), dim3(64 * W), (size_t)W * slot, stream, prm);
+    prm.n_units = n_units;
+    long grid;
+    if (prm.n_tiles % 8 == 0) {
+        prm.xcd_share = 0;
+        grid = (long)n_units * prm.n_tiles;
+    } else if (8 % prm.n_tiles == 0) {
+        prm.xcd_share = 8 / prm.n_tiles;
+        grid = 8L * ((n_units + prm.xcd_share - 1) / prm.xcd_share);
+    } else {
+        prm.xcd_share = -1;
+        grid = (long)n_units * prm.n_tiles;
+    }
+    if (grid > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: grid too large");
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64 * W), (size_t)W * slot, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
@@ -1122,9 +1204,17 @@ tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_
                                 hipStream_t stream)
 {
     if (K <= 0 || N <= 0) return TCA_OK;
-    hipLaunchKernelGGL(quantize_cols_kernel, dim3((N + 63) / 64), dim3(256), 0, stream, K, N, B, ldb, q, ldq, scale);
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, scale, 1, N, N);
+    TCA_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(quantize_cols_amax_kernel, dim3((N + 63) / 64, (K + kQuantRowsPerSlice - 1) / kQuantRowsPerSlice),
+                       dim3(256), 0, stream, K, N, B, ldb, scale);
+    TCA_HIP_TRY(hipGetLastError());
+    const long total = (long)K * N;
+    const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(quantize_cols_apply_kernel, dim3(blocks), dim3(256), 0, stream, K, N, B, ldb, q, ldq, scale);
+    TCA_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(quantize_cols_finalize_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, scale);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
-
 }  // namespace tca

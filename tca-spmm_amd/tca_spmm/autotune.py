@@ -37,8 +37,8 @@ def heuristic(M, K, nnz, N):
     return rpw, tile
 
 
-BATCH_RPW = (16, 32)          # variants compiled into the persistent batch kernel
-BATCH_TILES = (16, 32, 64)
+BATCH_RPW = (16,)             # variants compiled into the persistent batch kernel
+BATCH_TILES = {"spmm": (16, 32, 64), "spmm_t": (16, 32)}
 
 
 def choice_for(M, K, nnz, N, op="spmm", batch=False):
@@ -56,11 +56,11 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False):
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
         for c in best.get("candidates", [best]):
-            if not batch or (c["rows_per_wave"] in BATCH_RPW and c["tile_n"] in BATCH_TILES):
+            if not batch or (c["rows_per_wave"] in BATCH_RPW and c["tile_n"] in BATCH_TILES[op]):
                 return c["rows_per_wave"], c["tile_n"]
     rpw, tile = heuristic(M, K, nnz, N)
     if batch:
-        rpw, tile = min(rpw, 32), min(tile, 64)
+        rpw, tile = BATCH_RPW[0], min(tile, max(BATCH_TILES[op]))
     return rpw, tile
 
 
