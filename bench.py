@@ -127,14 +127,26 @@ def bench_fwd_bwd(torch, np, T, O, dev):
             T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
             T.spmm(L["plan_t"], L["w16"], L["G"], out=L["dX"], tile_n=L["tile_t"])
             T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
+    def only(op):
+        def run():
+            for L in layers:
+                if op == "fwd":
+                    T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
+                elif op == "dx":
+                    T.spmm(L["plan_t"], L["w16"], L["G"], out=L["dX"], tile_n=L["tile_t"])
+                else:
+                    T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
+        return run
     bt = T.Batch(items)
     plain_eager = time_eager(torch, per_call, 20)
     plain_graph = time_graph(torch, per_call, 20)
     fused = time_graph(torch, bt.run, 50)
+    per_op = {op: time_graph(torch, only(op), 20) for op in ("fwd", "dx", "dw")}
     return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph, "fused_ms": fused,
             "speedup_vs_eager": plain_eager / fused, "speedup_vs_graph": plain_graph / fused,
             "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "launches_plain": 3 * len(layers), "units": bt.n_units}
+            "per_call_graph_ms_by_op": per_op,
+            "launches_plain": 3 * len(layers), "units": bt.n_units, "workgroups": bt.n_workgroups}
 
 
 def bench_precision(torch, np, T, O, dev):
@@ -369,7 +381,17 @@ def main():
 
     # the same ten products as ONE persistent launch
     if "batched" in args.sections:
-        bt = T.Batch([T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], e["tile_n"]) for e in entries])
+        bitems = []
+        for e in entries:
+            A = e["case"].A
+            rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
+            if rpw_b == e["rpw"]:
+                bitems.append(T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], tile_b))
+            else:
+                pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
+                e["batch_keep"] = (pb, pb.pack(f16(A.values)))
+                bitems.append(T.spmm_item(pb, e["batch_keep"][1], e["B"], e["C"], tile_b))
+        bt = T.Batch(bitems)
         us = time_graph(torch, lambda: bt.run(), 100) * 1e3
         extra["batched_suite"] = {"us_per_launch": us, "tflops": total_flops / (us * 1e-6) / 1e12,
                                   "hbm_frac": total_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,

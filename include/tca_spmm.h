@@ -100,9 +100,11 @@ tca_status tca_f32_to_bf16_host(const float *in, size_t n, uint16_t *out);  /* r
 /* ------------------------------------------------------------------------ */
 /* Plan: the gfx950 panel tiling.  Replaces workload_manager                 */
 /* (src/our_spmm.cu:347-471), the reference's row->warp packing.  Rows are   */
-/* grouped `rows_per_wave` (16, 32 or 64) at a time; each group's nonzero     */
-/* columns are condensed and cut into 32-wide k-chunks; each (chunk, 16-row   */
-/* panel) lists its entries.  Values live outside the plan, in "tiled order" */
+/* grouped `rows_per_wave` (16, 32 or 64, one wave per group; or 128 / 256,  */
+/* one workgroup of rows/32 waves sharing each gathered B chunk) at a time;   */
+/* each group's nonzero columns are condensed and cut into 32-wide k-chunks;  */
+/* each (chunk, 16-row panel) lists its entries.  Values live outside the     */
+/* plan, in "tiled order" */
 /* (entry order); tca_pack_values converts from CSR order.  Built on the      */
 /* host from host CSR arrays; device tables go to the current HIP device.    */
 /* ------------------------------------------------------------------------ */
@@ -152,6 +154,7 @@ tca_status tca_unpack_values(const tca_plan *plan, tca_dtype dtype, const void *
 /*   row_scale[M] / col_scale[N] (may be NULL) multiply the FP32 result:     */
 /*   C[r,n] = row_scale[r] * col_scale[n] * sum_k A[r,k] B[k,n] (E4M3).      */
 /*   tile_n in {16, 32, 64, 128}, or 0 to pick from the built-in table.      */
+/*   Plans of 128 / 256 rows per group: tile_n 64 or 128, F16/BF16 A only.   */
 /* ------------------------------------------------------------------------ */
 tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled,
                     tca_dtype b_dtype, const void *d_B, int ldb, int N,

@@ -427,7 +427,7 @@ int orc_workload_manager(int M, const int *rowptr, int N, int warps_per_tb, int 
 int orc_tiling_sizes(int M, int K, const int *rowptr, const int *colidx, int rpw,
                      int *n_groups, int *n_chunks)
 {
-    if (rpw != 16 && rpw != 32 && rpw != 64) return -1;
+    if (rpw != 16 && rpw != 32 && rpw != 64 && rpw != 128 && rpw != 256) return -1;
     int G = (M + rpw - 1) / rpw;
     unsigned char *present = (unsigned char *)malloc((size_t)K);
     long total = 0;
@@ -450,7 +450,7 @@ int orc_tiling_fill(int M, int K, const int *rowptr, const int *colidx, int rpw,
                     int *grp_chunk_ptr, int *chunk_cols, int *seg_ptr,
                     uint16_t *entry_slot, int *entry_src, int *work_order)
 {
-    if (rpw != 16 && rpw != 32 && rpw != 64) return -1;
+    if (rpw != 16 && rpw != 32 && rpw != 64 && rpw != 128 && rpw != 256) return -1;
     int G = (M + rpw - 1) / rpw, ppg = rpw / 16;
     unsigned char *present = (unsigned char *)malloc((size_t)K);
     int *pos = (int *)malloc(sizeof(int) * (size_t)K);

@@ -497,7 +497,8 @@ namespace tca {
 tca_status build_tables(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx, int rpw,
                         PlanTables &t)
 {
-    if (rpw != 16 && rpw != 32 && rpw != 64) return fail(TCA_ERR_INVALID_ARG, "rows_per_wave must be 16, 32 or 64");
+    if (rpw != 16 && rpw != 32 && rpw != 64 && rpw != 128 && rpw != 256)
+        return fail(TCA_ERR_INVALID_ARG, "rows_per_wave must be 16, 32, 64, 128 or 256");
     if (n_rows <= 0 || n_cols <= 0 || nnz < 0 || !row_offset || (nnz > 0 && !col_idx))
         return fail(TCA_ERR_INVALID_ARG, "plan: bad CSR arguments");
     if (row_offset[0] != 0 || row_offset[n_rows] != nnz) return fail(TCA_ERR_FORMAT, "plan: row offsets do not span [0, nnz]");

@@ -516,6 +516,239 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide groups (rows_per_wave 128 / 256): one workgroup of WM waves owns a
+// (unit, N-tile) of a group of WM*PPW*16 rows, wave w owning panels
+// [w*PPW, (w+1)*PPW).  The waves share every gathered B chunk: the workgroup
+// loads the 32 x NT tile once (16-byte row pieces, one per thread and pass)
+// into LDS in row-major order with a pitch of NT+16 elements -- four
+// consecutive rows then start in distinct quarters of the 32 banks -- and
+// each wave reads its MFMA B operands with ds_read_b64_tr_b16: lane 4r+j of
+// a 16-lane group addresses row r, columns 4j..4j+3, and lane i receives
+// column i of the four rows, i.e. 4 consecutive k of one n.  Two reads (rows
+// 8q..8q+3 and 8q+4..8q+7 for lane group q) give the 8-k operand octet.
+// B tiles are double buffered with one workgroup barrier per chunk; B row
+// pieces, index words and panel entries are prefetched kWideDepth chunks
+// ahead in a register ring.  A tiles are per wave, as in spmm_unit.  Per
+// group and chunk the B rows are read once for all WM*PPW*16 rows instead of
+// once per 16..64-row wave, which is what the wide plans exist for.
+// ---------------------------------------------------------------------------
+constexpr int kWideDepth = 4;
+constexpr int kWideEP = 1;   // prefetched entries per lane per panel (64 of a 16x32 tile)
+
+template <int NT, int WM, int PPW>
+struct WideLds {
+    static constexpr int LDB = NT + 16;                          // B row pitch (elements)
+    static constexpr int BTILE = 32 * LDB * 2;                   // bytes per chunk tile
+    static constexpr int ATILE = PPW * 512 * 2;                  // bytes of one wave's A tiles
+    static constexpr int BYTES = 2 * BTILE + WM * ATILE;
+    static constexpr int SEGS = NT / 8;                          // 16-byte pieces per B row
+    static constexpr int PIECES = 32 * SEGS;
+    static constexpr int PPT = (PIECES + WM * 64 - 1) / (WM * 64);  // pieces per thread
+};
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint2 lds_read_tr16(const uint8_t *lds, int byte_off)
+{
+    const s16x4 v = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4 *)(lds + byte_off));
+    return __builtin_bit_cast(uint2, v);
+}
+
+template <int PPW, int NT, int WM, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+__global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
+{
+    using L = WideLds<NT, WM, PPW>;
+    using RL = RowLoad<AT, BT>;
+    using Oct = uint4;
+    using E = uint16_t;
+    constexpr int PPG = WM * PPW;       // panels of the plan's group
+    constexpr int RPG = PPG * 16;
+    constexpr int NSUB = NT / 16;
+    constexpr int D = kWideDepth;
+    constexpr int EP = kWideEP;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int tid = w * 64 + lane;
+    int ui, tile;
+    if (!xcd_work(p, blockIdx.x, ui, tile)) return;
+    const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
+    const int g = u.x, c_lo = u.z, c_hi = u.w;
+    const int n0 = tile * NT;
+    uint8_t *lds_a = smem + 2 * L::BTILE + w * L::ATILE;
+    const E *avals = static_cast<const E *>(p.a);
+    const int last_entry = p.nnz > 0 ? p.nnz - 1 : 0;
+
+    // this thread's B pieces: row k, 8-column segment
+    int pk[L::PPT], pseg[L::PPT];
+    bool pvalid[L::PPT];
+#pragma unroll
+    for (int j = 0; j < L::PPT; j++) {
+        const int pc = tid + j * WM * 64;
+        pvalid[j] = pc < L::PIECES;
+        pk[j] = pvalid[j] ? pc / L::SEGS : 0;
+        pseg[j] = pvalid[j] ? pc % L::SEGS : 0;
+    }
+    const int seg_lane = lane <= PPW ? lane : PPW;
+
+    struct Index {
+        int col[L::PPT];
+        int segb;
+    };
+    struct Stage {
+        typename RL::Raw rows[L::PPT];
+        uint32_t ok;
+        int slot[PPW][EP];
+        uint32_t val[PPW][EP];
+        int e0[PPW], e1[PPW];
+    };
+
+    f32x4 acc[PPW][NSUB];
+#pragma unroll
+    for (int i = 0; i < PPW; i++)
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) acc[i][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < PPW; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+
+    auto load_index = [&](int c, Index &ix) {
+#pragma unroll
+        for (int j = 0; j < L::PPT; j++) ix.col[j] = p.chunk_cols[(long)c * 32 + pk[j]];
+        ix.segb = p.seg_ptr[(long)c * PPG + w * PPW + seg_lane];
+    };
+    auto load_data = [&](const Index &ix, Stage &st) {
+        st.ok = 0;
+#pragma unroll
+        for (int j = 0; j < L::PPT; j++) {
+            const int ncol = n0 + 8 * pseg[j];
+            const bool ok = pvalid[j] && ix.col[j] >= 0 && ncol < p.N;
+            st.ok |= ok ? (1u << j) : 0u;
+            st.rows[j] = RL::load(p.B, ok ? (long)ix.col[j] * p.ldb + ncol : 0);
+        }
+#pragma unroll
+        for (int i = 0; i < PPW; i++) {
+            st.e0[i] = __shfl(ix.segb, i);
+            st.e1[i] = __shfl(ix.segb, i + 1);
+#pragma unroll
+            for (int k = 0; k < EP; k++) {
+                const int e = st.e0[i] + k * 64 + lane;
+                const bool ok = e < st.e1[i];
+                const int ec = ok ? e : last_entry;
+                const int sl = (int)p.entry_slot[ec];
+                st.slot[i][k] = ok ? sl : -1;
+                st.val[i][k] = INDIRECT ? (uint32_t)p.entry_src[ec] : (uint32_t)avals[ec];
+            }
+        }
+    };
+    auto a_off = [&](int i, int sl) { return (i * 64 + a_oct(sl >> 5, (sl >> 3) & 3)) * 16 + (sl & 7) * 2; };
+    // B operand addressing inside a tile (bytes): lane group q = lane/16, lane i = lane%16
+    const int brow = 8 * (lane >> 4) + ((lane & 15) >> 2);
+    const int bcol = 4 * (lane & 3);
+    auto put_b = [&](const Stage &st, uint8_t *buf) {
+#pragma unroll
+        for (int j = 0; j < L::PPT; j++)
+            if (pvalid[j]) {
+                const Oct v = (st.ok >> j) & 1u ? RL::cvt(st.rows[j]) : zero_oct<Oct>();
+                *reinterpret_cast<Oct *>(buf + (pk[j] * L::LDB + 8 * pseg[j]) * 2) = v;
+            }
+    };
+    auto compute = [&](const Stage &st, const uint8_t *buf) {
+        E v[PPW][EP];
+#pragma unroll
+        for (int i = 0; i < PPW; i++)
+#pragma unroll
+            for (int k = 0; k < EP; k++) v[i][k] = INDIRECT ? avals[st.val[i][k]] : (E)st.val[i][k];
+        bool overflow[PPW];
+#pragma unroll
+        for (int i = 0; i < PPW; i++) {
+#pragma unroll
+            for (int k = 0; k < EP; k++)
+                if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
+            overflow[i] = st.e1[i] - st.e0[i] > EP * 64;
+            if (overflow[i])
+                for (int e = st.e0[i] + EP * 64 + lane; e < st.e1[i]; e += 64) {
+                    const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
+                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_slot[e])) = vv;
+                }
+        }
+        wave_lds_sync();
+        Oct a[PPW];
+#pragma unroll
+        for (int i = 0; i < PPW; i++) a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            const int off = (brow * L::LDB + 16 * s + bcol) * 2;
+            const uint2 lo = lds_read_tr16(buf, off);
+            const uint2 hi = lds_read_tr16(buf, off + 4 * L::LDB * 2);
+            const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
+#pragma unroll
+            for (int i = 0; i < PPW; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < PPW; i++) {
+            if (overflow[i]) {
+                reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+            } else {
+#pragma unroll
+                for (int k = 0; k < EP; k++)
+                    if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
+            }
+        }
+    };
+
+    if (c_lo < c_hi) {
+        const int c_last = c_hi - 1;
+        Index ix[D];
+        Stage st[D];
+#pragma unroll
+        for (int s = 0; s < D; s++) load_index(min(c_lo + s, c_last), ix[s]);
+#pragma unroll
+        for (int s = 0; s < D; s++) {
+            load_data(ix[s], st[s]);
+            load_index(min(c_lo + s + D, c_last), ix[s]);
+        }
+        for (int c0 = c_lo; c0 < c_hi; c0 += D) {
+#pragma unroll
+            for (int s = 0; s < D; s++) {
+                const int c = c0 + s;
+                if (c < c_hi) {
+                    uint8_t *buf = smem + (c & 1) * L::BTILE;
+                    put_b(st[s], buf);
+                    __syncthreads();
+                    compute(st[s], buf);
+                }
+                load_data(ix[s], st[s]);
+                load_index(min(c + 2 * D, c_last), ix[s]);
+            }
+        }
+    }
+
+    // D[4*(lane/16)+j][lane%16] of each 16x16 tile -> C (atomic when the group is split)
+    const int rbase = g * RPG + w * PPW * 16 + 4 * (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < PPW; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = rbase + 16 * i + j;
+            if (row >= p.n_rows) continue;
+            const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+            float *crow = p.C + (long)row * p.ldc;
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                const int n = n0 + 16 * s + (lane & 15);
+                if (n < p.N) {
+                    float v = acc[i][s][j] * rs;
+                    if (p.col_scale) v *= p.col_scale[n];
+                    if (p.split) unsafeAtomicAdd(crow + n, v);
+                    else crow[n] = v;
+                }
+            }
+        }
+    }
+}
+
 using SpmmFn = void (*)(SpmmParams);
 
 template <int RPW, tca_dtype AT, tca_dtype BT, bool IND>
@@ -541,9 +774,36 @@ SpmmFn pick_rpw(int rpw, int nt)
     return nullptr;
 }
 
+template <int PPW, int WM, tca_dtype AT, tca_dtype BT, bool IND>
+SpmmFn pick_wide_nt(int nt)
+{
+    switch (nt) {
+    case 64: return spmm_wide_kernel<PPW, 64, WM, AT, BT, IND>;
+    case 128: return spmm_wide_kernel<PPW, 128, WM, AT, BT, IND>;
+    }
+    return nullptr;
+}
+
+template <tca_dtype AT, tca_dtype BT, bool IND>
+SpmmFn pick_wide_rpw(int rpw, int nt)
+{
+    switch (rpw) {
+    case 128: return pick_wide_nt<2, 4, AT, BT, IND>(nt);
+    case 256: return pick_wide_nt<2, 8, AT, BT, IND>(nt);
+    }
+    return nullptr;
+}
+
 template <bool IND>
 SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
 {
+    if (rpw >= 128) {
+        if (at == TCA_F16 && bt == TCA_F16) return pick_wide_rpw<TCA_F16, TCA_F16, IND>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_BF16) return pick_wide_rpw<TCA_BF16, TCA_BF16, IND>(rpw, nt);
+        if (at == TCA_F16 && bt == TCA_F32) return pick_wide_rpw<TCA_F16, TCA_F32, IND>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_F32) return pick_wide_rpw<TCA_BF16, TCA_F32, IND>(rpw, nt);
+        return nullptr;
+    }
     if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, IND>(rpw, nt);
     if (at == TCA_BF16 && bt == TCA_BF16) return pick_rpw<TCA_BF16, TCA_BF16, IND>(rpw, nt);
     if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_rpw<TCA_E4M3, TCA_E4M3, IND>(rpw, nt);
@@ -570,17 +830,19 @@ struct SddmmParams {
     int n_rows, N, ldg, ldx, rpw, n_blocks_per_group_max;
     const int32_t *unit_group;   // per workgroup: group id
     const int32_t *unit_chunk;   // per workgroup: first chunk
+    const int32_t *unit_panel;   // per workgroup: first panel (plans wider than the kernel), or null
 };
 
-// One wave = (group g, chunks [c0, c0+CB) of that group); `tile` is 16*33
-// floats of LDS.  The contraction over N runs as a two-stage register
+// One wave = (group g, chunks [c0, c0+CB) of that group, panels [p0, p0+RPW/16)
+// of the plan's rpw/16); `tile` is 16*33 floats of LDS.  The contraction over N runs as a two-stage register
 // pipeline: the G and X octets of step n+32 are loaded (unconditionally, with
 // row addresses clamped into range) while step n is multiplied.  Rows past the
 // matrix and padded columns only feed accumulator cells no entry samples.
 template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, float *tile)
+__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, int p0, float *tile)
 {
     constexpr int PPG = RPW / 16;
+    const int plan_ppg = p.rpw >> 4;
     using Oct = typename Elt<AT>::Oct;
     using GR = RowLoad<AT, GT>;
     using XR = RowLoad<AT, AT>;
@@ -612,7 +874,7 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
     long goff[PPG];
 #pragma unroll
     for (int i = 0; i < PPG; i++) {
-        const int row = g * RPW + 16 * i + (lane & 15);
+        const int row = g * p.rpw + 16 * (p0 + i) + (lane & 15);
         goff[i] = (long)(row < p.n_rows ? row : 0) * p.ldg;
     }
     const int q8 = 8 * (lane >> 4);
@@ -662,7 +924,7 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
                 for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][i][j];
             __syncthreads();
             const int c = c0 + b;
-            const int e0 = p.seg_ptr[c * PPG + i], e1 = p.seg_ptr[c * PPG + i + 1];
+            const int e0 = p.seg_ptr[c * plan_ppg + p0 + i], e1 = p.seg_ptr[c * plan_ppg + p0 + i + 1];
             for (int e = e0 + lane; e < e1; e += 64) {
                 const int slot = p.entry_slot[e];
                 p.out[e] = tile[(slot >> 5) * 33 + (slot & 31)];
@@ -676,7 +938,8 @@ template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
 __global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
 {
     __shared__ float tile[16 * 33];
-    sddmm_unit<RPW, CB, AT, GT>(p, p.unit_group[blockIdx.x], p.unit_chunk[blockIdx.x], tile);
+    sddmm_unit<RPW, CB, AT, GT>(p, p.unit_group[blockIdx.x], p.unit_chunk[blockIdx.x],
+                                p.unit_panel ? p.unit_panel[blockIdx.x] : 0, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -851,6 +1114,7 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     p.n_blocks_per_group_max = 0;
     p.unit_group = nullptr;
     p.unit_chunk = nullptr;
+    p.unit_panel = nullptr;
     return p;
 }
 
@@ -889,8 +1153,8 @@ __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restric
         switch (d.variant) {
             TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64)
             TCA_BWD_CASE(3, 16, 16) TCA_BWD_CASE(4, 16, 32)
-        case 5: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
-        case 6: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, reinterpret_cast<float *>(lds_a)); break;
+        case 5: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, 0, reinterpret_cast<float *>(lds_a)); break;
+        case 6: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, 0, reinterpret_cast<float *>(lds_a)); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next unit
@@ -968,7 +1232,7 @@ tca_status launch_warm_up(hipStream_t stream)
 
 int default_tile_n(const tca_plan *p, int N)
 {
-    (void)p;
+    if (p->host.rpw >= 128) return N >= 256 ? 128 : 64;
     if (N >= 128) return 64;
     if (N >= 32) return 32;
     return 16;
@@ -981,6 +1245,7 @@ int default_tile_n(const tca_plan *p, int N)
 // chunks per wave are split into S units that accumulate atomically.
 static constexpr int kTargetWaves = 4096;
 static constexpr int kMaxWaveChunks = 32;
+static constexpr long kWideTargetGroups = 1024;
 
 static int spmm_slot_bytes(int rpw, int nt, int es)
 {
@@ -1056,14 +1321,24 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.nnz = t.nnz;
     if (prm.n_tiles > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: too many N tiles");
 
-    const int es = a_dtype == TCA_E4M3 ? 1 : 2;
-    const int slot = spmm_slot_bytes(t.rpw, tile_n, es);
-    const int wcap = std::max(1, std::min(8, 65536 / slot));   // = SpmmLds<...>::WMAX
     const long base = (long)t.n_groups * prm.n_tiles;
     const int maxc = t.max_group_chunks;
-    int W = 1;
-    while (W * 2 <= wcap && maxc / (W * 2) >= 4 && base * W < kTargetWaves) W *= 2;
-    const int S = maxc > W * kMaxWaveChunks ? (maxc + W * kMaxWaveChunks - 1) / (W * kMaxWaveChunks) : 1;
+    int W = 1, S = 1;
+    size_t lds_bytes = 0;
+    if (t.rpw >= 128) {
+        // wide groups: W = rows / 32 waves share each B tile; split long groups
+        // until ~kWideTargetGroups workgroups exist (pieces of >= 8 chunks)
+        W = t.rpw / 32;
+        lds_bytes = 2 * 32 * (tile_n + 16) * 2 + W * 2 * 512 * 2;
+        if (base < kWideTargetGroups) S = (int)std::min<long>((kWideTargetGroups + base - 1) / base, std::max(1, maxc / 8));
+    } else {
+        const int es = a_dtype == TCA_E4M3 ? 1 : 2;
+        const int slot = spmm_slot_bytes(t.rpw, tile_n, es);
+        const int wcap = std::max(1, std::min(8, 65536 / slot));   // = SpmmLds<...>::WMAX
+        while (W * 2 <= wcap && maxc / (W * 2) >= 4 && base * W < kTargetWaves) W *= 2;
+        S = maxc > W * kMaxWaveChunks ? (maxc + W * kMaxWaveChunks - 1) / (W * kMaxWaveChunks) : 1;
+        lds_bytes = (size_t)W * slot;
+    }
 
     int n_units = 0;
     bool split = false;
@@ -1092,7 +1367,7 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
         grid = (long)n_units * prm.n_tiles;
     }
     if (grid > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: grid too large");
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64 * W), (size_t)W * slot, stream, prm);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64 * W), lds_bytes, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
@@ -1118,29 +1393,36 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
 {
     const PlanTables &t = p->host;
     SddmmFn fn = nullptr;
-    if (x_dtype == TCA_F16 && (g_dtype == TCA_F16)) fn = pick_sddmm_rpw<TCA_F16, TCA_F16>(t.rpw);
-    else if (x_dtype == TCA_F16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_F16, TCA_F32>(t.rpw);
-    else if (x_dtype == TCA_BF16 && g_dtype == TCA_BF16) fn = pick_sddmm_rpw<TCA_BF16, TCA_BF16>(t.rpw);
-    else if (x_dtype == TCA_BF16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_BF16, TCA_F32>(t.rpw);
+    const int kr = t.rpw <= 64 ? t.rpw : 32;
+    if (x_dtype == TCA_F16 && (g_dtype == TCA_F16)) fn = pick_sddmm_rpw<TCA_F16, TCA_F16>(kr);
+    else if (x_dtype == TCA_F16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_F16, TCA_F32>(kr);
+    else if (x_dtype == TCA_BF16 && g_dtype == TCA_BF16) fn = pick_sddmm_rpw<TCA_BF16, TCA_BF16>(kr);
+    else if (x_dtype == TCA_BF16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_BF16, TCA_F32>(kr);
     if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: unsupported dtype pair");
     if (p->transposed) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: needs a forward plan");
-    // work units: (group, first chunk) in heavy-first group order
+    // work units: (group, first chunk, first panel) in heavy-first group order;
+    // plans wider than 64 rows run 32-row panel pairs of a group per unit
+    const int krpw = t.rpw <= 64 ? t.rpw : 32;
     tca_plan *mp = const_cast<tca_plan *>(p);
     if (!mp->d_sddmm_units) {
-        std::vector<int32_t> ug, uc;
+        std::vector<int32_t> ug, uc, up;
         for (int gi = 0; gi < t.n_groups; gi++) {
             const int g = t.work_order[gi];
-            for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += kSddmmCB) {
-                ug.push_back(g);
-                uc.push_back(c);
-            }
+            for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += kSddmmCB)
+                for (int q = 0; q < t.rpw / krpw; q++) {
+                    if (g * t.rpw + q * krpw >= t.n_rows) break;
+                    ug.push_back(g);
+                    uc.push_back(c);
+                    up.push_back(q * krpw / 16);
+                }
         }
         mp->n_sddmm_units = (int)ug.size();
         if (mp->n_sddmm_units > 0) {
-            TCA_HIP_TRY(hipMalloc(&mp->d_sddmm_units, sizeof(int32_t) * 2 * ug.size()));
+            TCA_HIP_TRY(hipMalloc(&mp->d_sddmm_units, sizeof(int32_t) * 3 * ug.size()));
             int32_t *d = static_cast<int32_t *>(mp->d_sddmm_units);
             TCA_HIP_TRY(hipMemcpy(d, ug.data(), sizeof(int32_t) * ug.size(), hipMemcpyHostToDevice));
             TCA_HIP_TRY(hipMemcpy(d + ug.size(), uc.data(), sizeof(int32_t) * uc.size(), hipMemcpyHostToDevice));
+            TCA_HIP_TRY(hipMemcpy(d + 2 * ug.size(), up.data(), sizeof(int32_t) * up.size(), hipMemcpyHostToDevice));
         }
     }
     if (mp->n_sddmm_units == 0) return TCA_OK;
@@ -1160,6 +1442,7 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     prm.n_blocks_per_group_max = 0;
     prm.unit_group = static_cast<const int32_t *>(mp->d_sddmm_units);
     prm.unit_chunk = static_cast<const int32_t *>(mp->d_sddmm_units) + mp->n_sddmm_units;
+    prm.unit_panel = static_cast<const int32_t *>(mp->d_sddmm_units) + 2 * mp->n_sddmm_units;
     hipLaunchKernelGGL(fn, dim3(mp->n_sddmm_units), dim3(64), 0, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;

@@ -1,6 +1,7 @@
 """Autotuner for the two launch parameters of the panel SpMM:
-rows_per_wave (16/32/64 rows sharing one condensed column list) and
-tile_n (16/32/64/128 output columns per wave).
+rows_per_wave (16/32/64 rows sharing one condensed column list per wave, or
+128/256 rows per workgroup sharing each gathered B chunk) and tile_n
+(16/32/64/128 output columns; 64/128 for the 128/256-row groups).
 
 `tune()` times every candidate on the device with HIP events (launches queued
 behind a sleep so host enqueue cost stays off the clock) and returns them
@@ -14,6 +15,10 @@ import os
 
 RPW_CANDIDATES = (16, 32, 64)
 TILE_CANDIDATES = (16, 32, 64, 128)
+WIDE_RPW_CANDIDATES = (128, 256)
+WIDE_TILE_CANDIDATES = (64, 128)
+ALL_CANDIDATES = ([(r, t) for r in RPW_CANDIDATES for t in TILE_CANDIDATES]
+                  + [(r, t) for r in WIDE_RPW_CANDIDATES for t in WIDE_TILE_CANDIDATES])
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_table.json")
 
 _table = None
@@ -77,7 +82,7 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", tr
     if values_dev is None:
         values_dev = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(device)
     N = B_dev.shape[1]
-    cands = candidates or [(r, t) for r in RPW_CANDIDATES for t in TILE_CANDIDATES]
+    cands = candidates or ALL_CANDIDATES
     out = torch.empty((A.n_cols if transposed else A.n_rows, N), dtype=torch.float32, device=device)
     stream = torch.cuda.current_stream()
     res = []

@@ -44,7 +44,8 @@ class Plan:
     """Panel tiling of a CSR pattern (host tables + device copies).
 
     rows_per_wave in {16, 32, 64}: consecutive rows sharing one condensed
-    column list (16-row MFMA panels, 1/2/4 per wave).  device=False builds a
+    column list (16-row MFMA panels, 1/2/4 per wave); 128 or 256: one
+    workgroup of rows/32 waves (2 panels each) shares every gathered B chunk.  device=False builds a
     host-only plan (inspection/export only; launches reject it).
     """
 

@@ -162,7 +162,7 @@ def _cases():
     yield "empty_rows", (40, 33, np.array([0] * 20 + [0, 3] + [3] * 19, np.int32), np.array([1, 5, 32], np.int32))
 
 
-@pytest.mark.parametrize("rpw", [16, 32, 64])
+@pytest.mark.parametrize("rpw", [16, 32, 64, 128, 256])
 def test_tiling_bit_exact_against_oracle_rebuild(rpw):
     for name, (M, K, rp, ci) in _cases():
         plan = T.Plan(M, K, rp, ci, rows_per_wave=rpw, device=False)
