@@ -40,12 +40,14 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         d = tca::BatchItemDev{};
         d.n_rows = t.n_rows;
         d.nnz = t.nnz;
+        d.n_chunks = t.n_chunks;
         d.N = it.N;
         d.rpw = t.rpw;
         d.grp_chunk_ptr = p->d_grp_chunk_ptr;
         d.chunk_cols = p->d_chunk_cols;
         d.seg_ptr = p->d_seg_ptr;
         d.entry_slot = p->d_entry_slot;
+        d.entry_lds = p->d_entry_lds;
         d.entry_src = p->d_entry_src;
         d.a = it.a;
         d.b = it.b;
@@ -63,6 +65,9 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;
+            d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, it.b_dtype == TCA_F32 ? 4 : 2);
+            if (d.b_bytes == 0 || (long)t.nnz * 4 >= (1L << 31))
+                return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " reaches 2 GiB");
             for (int gi = 0; gi < t.n_groups; gi++) {
                 const int g = t.work_order[gi];
                 const int nch = t.grp_chunk_ptr[g + 1] - t.grp_chunk_ptr[g];
@@ -82,6 +87,10 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                                                      " (batched SDDMM: rows_per_wave 16, f16 X, f32 or f16 G)");
             d.ldb = it.ldb;   // X
             d.ldc = it.lda;   // G
+            d.a_bytes = tca::operand_bytes(t.n_rows, it.N, it.lda, it.a_dtype == TCA_F32 ? 4 : 2);
+            d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, 2);
+            if (d.a_bytes == 0 || d.b_bytes == 0)
+                return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) + " reaches 2 GiB");
             for (int gi = 0; gi < t.n_groups; gi++) {
                 const int g = t.work_order[gi];
                 for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {

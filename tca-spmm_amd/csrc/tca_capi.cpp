@@ -2,6 +2,8 @@
 // points.  Argument checking happens here; kernels assume valid arguments.
 #include "tca_internal.hpp"
 
+#include <vector>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -13,20 +15,33 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Element index of panel-tile slot (r16 << 5) | k in the SpMM kernels' LDS A
+// tile: 16 rows x 32 k stored as octets, two rows per 128-byte line, octet
+// slot XOR-swizzled by the row pair (the device function a_oct); precomputed
+// here so that the kernels scatter entries without address arithmetic.
+uint16_t a_tile_index(uint16_t slot)
+{
+    const int r = slot >> 5, q = (slot >> 3) & 3, m = r >> 1;
+    return (uint16_t)((m * 8 + ((((r & 1) << 2) | q) ^ (m & 3))) * 8 + (slot & 7));
+}
+
 tca_status upload(tca_plan *p)
 {
     const tca::PlanTables &t = p->host;
     const int ppg = t.rpw / 16;
-    const size_t sz[6] = {
+    std::vector<uint16_t> lds(t.entry_slot.size());
+    for (size_t e = 0; e < lds.size(); e++) lds[e] = a_tile_index(t.entry_slot[e]);
+    const size_t sz[7] = {
         sizeof(int32_t) * (size_t)(t.n_groups + 1),
         sizeof(int32_t) * (size_t)t.n_chunks * 32,
         sizeof(int32_t) * ((size_t)t.n_chunks * ppg + 1),
         sizeof(uint16_t) * (size_t)t.nnz,
         sizeof(int32_t) * (size_t)t.nnz,
         sizeof(int32_t) * (size_t)t.n_groups,
+        sizeof(uint16_t) * (size_t)t.nnz,
     };
-    size_t off[6], total = 0;
-    for (int i = 0; i < 6; i++) {
+    size_t off[7], total = 0;
+    for (int i = 0; i < 7; i++) {
         off[i] = total;
         total = align_up(total + std::max<size_t>(sz[i], 4), 256);
     }
@@ -36,9 +51,9 @@ tca_status upload(tca_plan *p)
     p->d_base = base;
     p->d_bytes = total;
     char *b = static_cast<char *>(base);
-    const void *src[6] = {t.grp_chunk_ptr.data(), t.chunk_cols.data(), t.seg_ptr.data(),
-                          t.entry_slot.data(),    t.entry_src.data(),  t.work_order.data()};
-    for (int i = 0; i < 6; i++)
+    const void *src[7] = {t.grp_chunk_ptr.data(), t.chunk_cols.data(), t.seg_ptr.data(), t.entry_slot.data(),
+                          t.entry_src.data(),     t.work_order.data(), lds.data()};
+    for (int i = 0; i < 7; i++)
         if (sz[i]) TCA_HIP_TRY(hipMemcpy(b + off[i], src[i], sz[i], hipMemcpyHostToDevice));
     p->d_grp_chunk_ptr = reinterpret_cast<const int32_t *>(b + off[0]);
     p->d_chunk_cols = reinterpret_cast<const int32_t *>(b + off[1]);
@@ -46,6 +61,7 @@ tca_status upload(tca_plan *p)
     p->d_entry_slot = reinterpret_cast<const uint16_t *>(b + off[3]);
     p->d_entry_src = reinterpret_cast<const int32_t *>(b + off[4]);
     p->d_work_order = reinterpret_cast<const int32_t *>(b + off[5]);
+    p->d_entry_lds = reinterpret_cast<const uint16_t *>(b + off[6]);
     return TCA_OK;
 }
 

@@ -54,6 +54,7 @@ struct tca_plan {
     const int32_t *d_chunk_cols = nullptr;
     const int32_t *d_seg_ptr = nullptr;
     const uint16_t *d_entry_slot = nullptr;
+    const uint16_t *d_entry_lds = nullptr;   // entry_slot as an index into the SpMM LDS A tile
     const int32_t *d_entry_src = nullptr;
     const int32_t *d_work_order = nullptr;
     // SDDMM work units (group, first chunk), built on first use
@@ -86,13 +87,18 @@ tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_
                                 hipStream_t stream);
 int default_tile_n(const tca_plan *p, int N);
 tca_status launch_warm_up(hipStream_t stream);
+// Bytes from the first to the last element of a rows x N row-major operand with
+// leading dimension ld; 0 when that is not below 2^31 (the kernels address
+// operands with 32-bit byte offsets).
+uint32_t operand_bytes(long rows, long N, long ld, int elem_bytes);
 
 // Batched persistent launch: one descriptor per item, one record per work unit.
 struct BatchItemDev {
     int32_t variant;   // kernel variant id (batch_variant)
-    int32_t n_rows, N, ldb, ldc, n_tiles, rpw, nnz;
+    int32_t n_rows, N, ldb, ldc, n_tiles, rpw, nnz, n_chunks;
+    uint32_t a_bytes, b_bytes;   // SPMM: -, B; SDDMM: G, X (bytes from first to last used element)
     const int32_t *grp_chunk_ptr, *chunk_cols, *seg_ptr;
-    const uint16_t *entry_slot;
+    const uint16_t *entry_slot, *entry_lds;
     const int32_t *entry_src;
     const void *a, *b;
     float *out;

@@ -120,6 +120,61 @@ template <> struct RowLoad<TCA_BF16, TCA_F32> {
 };
 
 // ---------------------------------------------------------------------------
+// Buffer-resource loads: 32-bit byte offsets from a uniform base, and reads at
+// or past `bytes` return zeros.  Padding / masked lanes pass kOOB instead of a
+// clamped address, so no select or 64-bit address arithmetic is spent on them.
+// Every buffer addressed this way is below 2^31 bytes (checked on the host).
+// ---------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kOOB = 0x80000000u;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint4 buf_b128(Rsrc r, uint32_t off, uint32_t soff = 0)
+{
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0));
+}
+__device__ __forceinline__ uint2 buf_b64(Rsrc r, uint32_t off, uint32_t soff = 0)
+{
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
+}
+__device__ __forceinline__ uint32_t buf_b32(Rsrc r, uint32_t off, uint32_t soff = 0)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0);
+}
+__device__ __forceinline__ uint32_t buf_b16(Rsrc r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0); }
+__device__ __forceinline__ uint32_t buf_b8(Rsrc r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0); }
+
+// 8 consecutive elements of a dense row by buffer load (byte offset; kOOB -> zeros)
+template <tca_dtype AT, tca_dtype BT> struct RowBuf;
+template <tca_dtype AT> struct RowBuf<AT, AT> {
+    static constexpr int BES = Elt<AT>::ES;          // bytes per stored element
+    using Raw = typename Elt<AT>::Oct;
+    __device__ static Raw load(Rsrc r, uint32_t off, uint32_t soff = 0)
+    {
+        if constexpr (BES == 2) return buf_b128(r, off, soff);
+        else return buf_b64(r, off, soff);
+    }
+};
+template <tca_dtype AT> struct RowBuf<AT, TCA_F32> {
+    static constexpr int BES = 4;
+    using Raw = F32x8;
+    __device__ static Raw load(Rsrc r, uint32_t off, uint32_t soff = 0)
+    {
+        return F32x8{__builtin_bit_cast(float4, buf_b128(r, off, soff)), __builtin_bit_cast(float4, buf_b128(r, off + 16, soff))};
+    }
+};
+template <tca_dtype AT> __device__ __forceinline__ uint32_t load_value(Rsrc r, uint32_t idx)
+{
+    if constexpr (Elt<AT>::ES == 2) return buf_b16(r, idx * 2);
+    else return buf_b8(r, idx);
+}
+
+// ---------------------------------------------------------------------------
 // LDS layouts (offsets in octets).  B tile: NT rows (n) x 32 k; two n-rows
 // share a 128-byte line of 8 octet slots, XOR-swizzled so that the 8 lanes of
 // an MFMA operand read (n = 16s+0..7, fixed q) and the 16 lanes of a
@@ -137,6 +192,22 @@ __device__ inline int a_oct(int r, int q)
     const int m = r >> 1;
     return m * 8 + ((((r & 1) << 2) | q) ^ (m & 3));
 }
+
+// Transposing LDS read (ds_read_b64_tr_b16): lane 4r+j of each 16-lane group
+// addresses row r, columns 4j..4j+3 of a row-major 16-bit tile, and lane i of
+// the group receives column i of the four rows -- 4 consecutive k of one n,
+// half of an MFMA B operand octet.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint2 lds_read_tr16(const uint8_t *lds, int byte_off)
+{
+    const s16x4 v = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4 *)(lds + byte_off));
+    return __builtin_bit_cast(uint2, v);
+}
+
+// Row pitch (elements) of a row-major 32 x NT 16-bit B tile: 4 consecutive rows
+// start in distinct quarters of the 32 banks (pitch = 32 or 96 bytes mod 128).
+template <int NT> constexpr int tr_pitch() { return NT == 16 ? 48 : NT + 16; }
 
 // Orders one wave's LDS accesses (write -> read -> write of its own tiles).
 // LDS instructions of a wave execute in issue order, so only the compiler
@@ -163,6 +234,7 @@ struct SpmmParams {
     const int32_t *chunk_cols;
     const int32_t *seg_ptr;
     const uint16_t *entry_slot;
+    const uint16_t *entry_lds;   // entry_slot as element index into the LDS A tile (a_oct layout)
     const int32_t *entry_src;
     const int32_t *work_order;
     const void *a;
@@ -174,6 +246,8 @@ struct SpmmParams {
     int n_rows, N, ldb, ldc, n_tiles, n_groups, rpw, split, nnz;
     int n_units;             // standalone launch: entries of `units`
     int xcd_share;           // standalone launch: XCDs sharing one N-tile (see xcd_work)
+    uint32_t b_bytes;        // bytes of B from its first to its last used element
+    int n_chunks;            // chunks of the plan (chunk_cols holds 32 per chunk)
 };
 
 // Workgroup -> (unit, N-tile) for a 1-D standalone grid.  Workgroups are
@@ -239,7 +313,7 @@ constexpr int kEP = 2;
 constexpr int kDepth = 2;
 
 // One wave = one (group g, N-tile `tile`, chunks [c_lo, c_hi) of g).  RPW rows
-// (PPG = RPW/16 panels) x NT columns; lds_b holds NT*32 elements, lds_a
+// (PPG = RPW/16 panels) x NT columns; lds_b holds the 32 x NT B tile, lds_a
 // PPG*512 elements.  Software pipeline over a ring of kDepth register stages
 // (indexed by compile-time slot, so nothing is copied between iterations):
 //   index loads (column ids, segment bounds) run 2*kDepth chunks ahead,
@@ -259,6 +333,10 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     constexpr int KPL = 32 / LPS;      // consecutive k rows per lane (= NT/16)
     constexpr int OCT = 8 * ES;        // octet bytes
     constexpr int D = kDepth;
+    // 16-bit types: B tile row-major [32 k][LDB], operands by transposing reads;
+    // FP8: B tile [n][32 k] octets written k-packed (pack_k), plain operand reads
+    constexpr bool TR = ES == 2;
+    constexpr int LDB = tr_pitch<NT>();
     using Oct = typename Elt<AT>::Oct;
     using E = typename Store<ES>::T;
 
@@ -267,11 +345,13 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         int segb;                      // lane j <= PPG: seg_ptr[c*PPG + j]
     };
     using RL = RowLoad<AT, BT>;
+    using RB = RowBuf<AT, BT>;
     struct Stage {
-        typename RL::Raw rows[KPL];
-        uint32_t rows_ok;              // bit j: rows[j] is a real B row (else zeros)
-        int slot[PPG][kEP];            // -1: no entry
+        typename RB::Raw rows[KPL];    // padded k rows and tail columns read as zeros
+        int slot[PPG][kEP];            // raw entry slot (valid where eok has the bit)
+        uint32_t eok;                  // bit i*kEP+k: slot[i][k] is a real entry
         uint32_t val[PPG][kEP];        // value bits, or entry_src when INDIRECT
+        uint32_t vv[PPG][kEP];         // INDIRECT: value bits, loaded one chunk ahead
         int e0[PPG], e1[PPG];
     };
 
@@ -290,28 +370,41 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     const bool col_ok = ncol < p.N;
     const E *avals = static_cast<const E *>(p.a);
     const int seg_lane = lane <= PPG ? lane : PPG;
-    const int last_entry = p.nnz > 0 ? p.nnz - 1 : 0;
+    const uint32_t nnz = (uint32_t)p.nnz;
+    const Rsrc r_b = make_rsrc(p.B, p.b_bytes);
+    const Rsrc r_cols = make_rsrc(p.chunk_cols, (uint32_t)p.n_chunks * 128u);
+    const Rsrc r_lds = make_rsrc(p.entry_lds, nnz * 2u);
+    const Rsrc r_val = make_rsrc(p.a, nnz * (uint32_t)ES);
+    const Rsrc r_src = make_rsrc(p.entry_src, nnz * 4u);
+    const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
+    const uint32_t col_bytes = (uint32_t)ncol * RB::BES;
 
 #pragma unroll
     for (int i = 0; i < PPG; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
 
     auto load_index = [&](int c, Index &ix) {
-        const int32_t *cc = p.chunk_cols + (long)c * 32 + k0;
+        const uint32_t cb = ((uint32_t)c * 32u + (uint32_t)k0) * 4u;
+        if constexpr (KPL == 4) {
+            const uint4 v = buf_b128(r_cols, cb);
+            ix.cols[0] = (int)v.x; ix.cols[1] = (int)v.y; ix.cols[2] = (int)v.z; ix.cols[3] = (int)v.w;
+        } else if constexpr (KPL == 2) {
+            const uint2 v = buf_b64(r_cols, cb);
+            ix.cols[0] = (int)v.x; ix.cols[1] = (int)v.y;
+        } else {
 #pragma unroll
-        for (int j = 0; j < KPL; j++) ix.cols[j] = cc[j];
+            for (int j = 0; j < KPL; j++) ix.cols[j] = (int)buf_b32(r_cols, cb + 4u * j);
+        }
         ix.segb = p.seg_ptr[c * PPG + seg_lane];
     };
-    // Loads are issued unconditionally (addresses clamped into range, results
-    // masked when used) so that the waits the compiler places stay counted
-    // per load instead of draining the whole queue at each branch.
+    // Loads are issued unconditionally, results are only used at compute time:
+    // masking a loaded value here would wait for it (and, the counter being in
+    // order, for every load issued before it).
     auto load_data = [&](const Index &ix, Stage &st) {
-        st.rows_ok = 0;
+        st.eok = 0;
 #pragma unroll
         for (int j = 0; j < KPL; j++) {
             const bool ok = ix.cols[j] >= 0 && col_ok;
-            st.rows_ok |= ok ? (1u << j) : 0u;
-            const long off = ok ? (long)ix.cols[j] * p.ldb + ncol : 0;
-            st.rows[j] = RL::load(p.B, off);
+            st.rows[j] = RB::load(r_b, ok ? (uint32_t)ix.cols[j] * ldb_bytes + col_bytes : kOOB);
         }
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
@@ -321,40 +414,55 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             for (int k = 0; k < kEP; k++) {
                 const int e = st.e0[i] + k * 64 + lane;
                 const bool ok = e < st.e1[i];
-                const int ec = ok ? e : last_entry;
-                const int sl = (int)p.entry_slot[ec];
-                st.slot[i][k] = ok ? sl : -1;
-                st.val[i][k] = INDIRECT ? (uint32_t)p.entry_src[ec] : (uint32_t)avals[ec];
+                const uint32_t eo = ok ? (uint32_t)e : (kOOB >> 2);
+                st.slot[i][k] = (int)buf_b16(r_lds, eo * 2u);
+                st.eok |= ok ? 1u << (i * kEP + k) : 0u;
+                st.val[i][k] = INDIRECT ? buf_b32(r_src, eo * 4u) : load_value<AT>(r_val, eo);
             }
         }
     };
-    auto a_off = [&](int i, int s) { return (i * 64 + a_oct(s >> 5, (s >> 3) & 3)) * OCT + (s & 7) * ES; };
+    // INDIRECT: the values behind entry_src, issued one chunk before use (the
+    // entry_src words came a chunk earlier still, so the wait is partial)
+    auto load_vals = [&](Stage &st) {
+        if constexpr (INDIRECT) {
+#pragma unroll
+            for (int i = 0; i < PPG; i++)
+#pragma unroll
+                for (int k = 0; k < kEP; k++) st.vv[i][k] = load_value<AT>(r_val, st.val[i][k]);
+        }
+    };
+    auto a_off = [&](int i, int x) { return (i * 512 + x) * ES; };   // x: entry_lds index
     auto compute = [&](const Stage &st) {
         E v[PPG][kEP];
 #pragma unroll
         for (int i = 0; i < PPG; i++)
 #pragma unroll
-            for (int k = 0; k < kEP; k++) v[i][k] = INDIRECT ? avals[st.val[i][k]] : (E)st.val[i][k];
+            for (int k = 0; k < kEP; k++) v[i][k] = INDIRECT ? (E)st.vv[i][k] : (E)st.val[i][k];
         Oct rows[KPL];
 #pragma unroll
-        for (int j = 0; j < KPL; j++) rows[j] = (st.rows_ok >> j) & 1u ? RL::cvt(st.rows[j]) : zero_oct<Oct>();
+        for (int j = 0; j < KPL; j++) rows[j] = RL::cvt(st.rows[j]);
+        if constexpr (TR) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int n = 8 * seg + e;
-            const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
-            *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
+            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = rows[j];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int n = 8 * seg + e;
+                const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
+                *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
+            }
         }
         bool overflow[PPG];
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
 #pragma unroll
             for (int k = 0; k < kEP; k++)
-                if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
+                if ((st.eok >> (i * kEP + k)) & 1u) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
             overflow[i] = st.e1[i] - st.e0[i] > kEP * 64;
             if (overflow[i])
                 for (int e = st.e0[i] + kEP * 64 + lane; e < st.e1[i]; e += 64) {
                     const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_slot[e])) = vv;
+                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_lds[e])) = vv;
                 }
         }
         wave_lds_sync();
@@ -363,7 +471,15 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int i = 0; i < PPG; i++) a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
-            const Oct b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
+            Oct b;
+            if constexpr (TR) {
+                const int off = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
+                const uint2 lo = lds_read_tr16(lds_b, off);
+                const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+                b = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
+            }
 #pragma unroll
             for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
         }
@@ -376,7 +492,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             } else {
 #pragma unroll
                 for (int k = 0; k < kEP; k++)
-                    if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
+                    if ((st.eok >> (i * kEP + k)) & 1u) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
             }
         }
     };
@@ -394,6 +510,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             load_data(ix[s], st[s]);
             load_index(min(c_lo + s + D, c_last), ix[s]);
         }
+        load_vals(st[0]);
         for (int c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
             for (int s = 0; s < D; s++) {
@@ -401,6 +518,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
                 if (c < c_hi) compute(st[s]);
                 load_data(ix[s], st[s]);
                 load_index(min(c + 2 * D, c_last), ix[s]);
+                load_vals(st[(s + 1) % D]);
             }
         }
     }
@@ -442,7 +560,7 @@ __device__ __forceinline__ void spmm_store_direct(const SpmmParams &p, int g, in
 // loop, then its partial RPW x NT tile (rows padded by 4 floats).
 template <int RPW, int NT, tca_dtype AT>
 struct SpmmLds {
-    static constexpr int B = NT * 32 * Elt<AT>::ES;
+    static constexpr int B = Elt<AT>::ES == 2 ? 32 * tr_pitch<NT>() * 2 : NT * 32 * Elt<AT>::ES;
     static constexpr int A = (RPW / 16) * 512 * Elt<AT>::ES;
     static constexpr int RED_LD = NT + 4;
     static constexpr int RED = RPW * RED_LD * 4;
@@ -534,7 +652,7 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
 // once per 16..64-row wave, which is what the wide plans exist for.
 // ---------------------------------------------------------------------------
 constexpr int kWideDepth = 4;
-constexpr int kWideEP = 1;   // prefetched entries per lane per panel (64 of a 16x32 tile)
+constexpr int kWideEP = 2;   // prefetched entries per lane per panel (128 of a 16x32 tile)
 
 template <int NT, int WM, int PPW>
 struct WideLds {
@@ -547,19 +665,12 @@ struct WideLds {
     static constexpr int PPT = (PIECES + WM * 64 - 1) / (WM * 64);  // pieces per thread
 };
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint2 lds_read_tr16(const uint8_t *lds, int byte_off)
-{
-    const s16x4 v = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4 *)(lds + byte_off));
-    return __builtin_bit_cast(uint2, v);
-}
-
 template <int PPW, int NT, int WM, tca_dtype AT, tca_dtype BT, bool INDIRECT>
 __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 {
     using L = WideLds<NT, WM, PPW>;
     using RL = RowLoad<AT, BT>;
+    using RB = RowBuf<AT, BT>;
     using Oct = uint4;
     using E = uint16_t;
     constexpr int PPG = WM * PPW;       // panels of the plan's group
@@ -578,7 +689,12 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     const int n0 = tile * NT;
     uint8_t *lds_a = smem + 2 * L::BTILE + w * L::ATILE;
     const E *avals = static_cast<const E *>(p.a);
-    const int last_entry = p.nnz > 0 ? p.nnz - 1 : 0;
+    const uint32_t nnz = (uint32_t)p.nnz;
+    const Rsrc r_b = make_rsrc(p.B, p.b_bytes);
+    const Rsrc r_lds = make_rsrc(p.entry_lds, nnz * 2u);
+    const Rsrc r_val = make_rsrc(p.a, nnz * 2u);
+    const Rsrc r_src = make_rsrc(p.entry_src, nnz * 4u);
+    const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
 
     // this thread's B pieces: row k, 8-column segment
     int pk[L::PPT], pseg[L::PPT];
@@ -597,10 +713,11 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         int segb;
     };
     struct Stage {
-        typename RL::Raw rows[L::PPT];
-        uint32_t ok;
-        int slot[PPW][EP];
-        uint32_t val[PPW][EP];
+        typename RB::Raw rows[L::PPT];  // absent rows / tail columns read as zeros
+        int slot[PPW][EP];             // raw entry slot (valid where eok has the bit)
+        uint32_t eok;
+        uint32_t val[PPW][EP];         // value bits, or entry_src when INDIRECT
+        uint32_t vv[PPW][EP];          // INDIRECT: value bits, loaded one chunk ahead
         int e0[PPW], e1[PPW];
     };
 
@@ -618,13 +735,12 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         ix.segb = p.seg_ptr[(long)c * PPG + w * PPW + seg_lane];
     };
     auto load_data = [&](const Index &ix, Stage &st) {
-        st.ok = 0;
+        st.eok = 0;
 #pragma unroll
         for (int j = 0; j < L::PPT; j++) {
             const int ncol = n0 + 8 * pseg[j];
             const bool ok = pvalid[j] && ix.col[j] >= 0 && ncol < p.N;
-            st.ok |= ok ? (1u << j) : 0u;
-            st.rows[j] = RL::load(p.B, ok ? (long)ix.col[j] * p.ldb + ncol : 0);
+            st.rows[j] = RB::load(r_b, ok ? (uint32_t)ix.col[j] * ldb_bytes + (uint32_t)ncol * RB::BES : kOOB);
         }
 #pragma unroll
         for (int i = 0; i < PPW; i++) {
@@ -634,14 +750,22 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             for (int k = 0; k < EP; k++) {
                 const int e = st.e0[i] + k * 64 + lane;
                 const bool ok = e < st.e1[i];
-                const int ec = ok ? e : last_entry;
-                const int sl = (int)p.entry_slot[ec];
-                st.slot[i][k] = ok ? sl : -1;
-                st.val[i][k] = INDIRECT ? (uint32_t)p.entry_src[ec] : (uint32_t)avals[ec];
+                const uint32_t eo = ok ? (uint32_t)e : (kOOB >> 2);
+                st.slot[i][k] = (int)buf_b16(r_lds, eo * 2u);
+                st.eok |= ok ? 1u << (i * EP + k) : 0u;
+                st.val[i][k] = INDIRECT ? buf_b32(r_src, eo * 4u) : buf_b16(r_val, eo * 2u);
             }
         }
     };
-    auto a_off = [&](int i, int sl) { return (i * 64 + a_oct(sl >> 5, (sl >> 3) & 3)) * 16 + (sl & 7) * 2; };
+    auto load_vals = [&](Stage &st) {
+        if constexpr (INDIRECT) {
+#pragma unroll
+            for (int i = 0; i < PPW; i++)
+#pragma unroll
+                for (int k = 0; k < EP; k++) st.vv[i][k] = buf_b16(r_val, st.val[i][k] * 2u);
+        }
+    };
+    auto a_off = [&](int i, int x) { return (i * 512 + x) * 2; };   // x: entry_lds index
     // B operand addressing inside a tile (bytes): lane group q = lane/16, lane i = lane%16
     const int brow = 8 * (lane >> 4) + ((lane & 15) >> 2);
     const int bcol = 4 * (lane & 3);
@@ -649,7 +773,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 #pragma unroll
         for (int j = 0; j < L::PPT; j++)
             if (pvalid[j]) {
-                const Oct v = (st.ok >> j) & 1u ? RL::cvt(st.rows[j]) : zero_oct<Oct>();
+                const Oct v = RL::cvt(st.rows[j]);
                 *reinterpret_cast<Oct *>(buf + (pk[j] * L::LDB + 8 * pseg[j]) * 2) = v;
             }
     };
@@ -658,18 +782,18 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 #pragma unroll
         for (int i = 0; i < PPW; i++)
 #pragma unroll
-            for (int k = 0; k < EP; k++) v[i][k] = INDIRECT ? avals[st.val[i][k]] : (E)st.val[i][k];
+            for (int k = 0; k < EP; k++) v[i][k] = INDIRECT ? (E)st.vv[i][k] : (E)st.val[i][k];
         bool overflow[PPW];
 #pragma unroll
         for (int i = 0; i < PPW; i++) {
 #pragma unroll
             for (int k = 0; k < EP; k++)
-                if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
+                if ((st.eok >> (i * EP + k)) & 1u) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = v[i][k];
             overflow[i] = st.e1[i] - st.e0[i] > EP * 64;
             if (overflow[i])
                 for (int e = st.e0[i] + EP * 64 + lane; e < st.e1[i]; e += 64) {
                     const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_slot[e])) = vv;
+                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_lds[e])) = vv;
                 }
         }
         wave_lds_sync();
@@ -693,7 +817,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             } else {
 #pragma unroll
                 for (int k = 0; k < EP; k++)
-                    if (st.slot[i][k] >= 0) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
+                    if ((st.eok >> (i * EP + k)) & 1u) *reinterpret_cast<E *>(lds_a + a_off(i, st.slot[i][k])) = (E)0;
             }
         }
     };
@@ -709,6 +833,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             load_data(ix[s], st[s]);
             load_index(min(c_lo + s + D, c_last), ix[s]);
         }
+        load_vals(st[0]);
         for (int c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
             for (int s = 0; s < D; s++) {
@@ -721,6 +846,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
                 }
                 load_data(ix[s], st[s]);
                 load_index(min(c + 2 * D, c_last), ix[s]);
+                load_vals(st[(s + 1) % D]);
             }
         }
     }
@@ -831,31 +957,32 @@ struct SddmmParams {
     const int32_t *unit_group;   // per workgroup: group id
     const int32_t *unit_chunk;   // per workgroup: first chunk
     const int32_t *unit_panel;   // per workgroup: first panel (plans wider than the kernel), or null
+    uint32_t g_bytes, x_bytes;   // bytes of G and X from the first to the last used element
 };
 
-// One wave = (group g, chunks [c0, c0+CB) of that group, panels [p0, p0+RPW/16)
-// of the plan's rpw/16); `tile` is 16*33 floats of LDS.  The contraction over N runs as a two-stage register
-// pipeline: the G and X octets of step n+32 are loaded (unconditionally, with
-// row addresses clamped into range) while step n is multiplied.  Rows past the
-// matrix and padded columns only feed accumulator cells no entry samples.
+// One wave accumulates the 16x32 result tiles of (group g, chunks
+// [c0, c0+nc), panels [p0, p0+RPW/16) of the plan's rpw/16) over the
+// contraction range [n_lo, n_hi) (multiples of 32, n_hi > n_lo).  The
+// contraction runs as a two-stage register pipeline: the G and X octets of
+// step n+32 are loaded (unconditionally, with row addresses clamped into
+// range) while step n is multiplied.  Rows past the matrix and padded columns
+// only feed accumulator cells no entry samples.
 template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, int p0, float *tile)
+__device__ __forceinline__ void sddmm_acc(const SddmmParams &p, int g, int c0, int p0, int nc, int n_lo, int n_hi,
+                                          f32x4 (&acc)[CB][2][RPW / 16])
 {
     constexpr int PPG = RPW / 16;
-    const int plan_ppg = p.rpw >> 4;
     using Oct = typename Elt<AT>::Oct;
     using GR = RowLoad<AT, GT>;
     using XR = RowLoad<AT, AT>;
+    using GB = RowBuf<AT, GT>;
+    using XB = RowBuf<AT, AT>;
     struct Step {
-        typename GR::Raw g[PPG];
-        typename XR::Raw x[CB][2];
+        typename GB::Raw g[PPG];
+        typename XB::Raw x[CB][2];
     };
 
     const int lane = lane_id();
-    const int c1 = min(c0 + CB, p.grp_chunk_ptr[g + 1]);
-    const int nc = c1 - c0;
-
-    f32x4 acc[CB][2][PPG];
 #pragma unroll
     for (int b = 0; b < CB; b++)
 #pragma unroll
@@ -863,28 +990,32 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
 #pragma unroll
             for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    long xoff[CB][2];   // X row of chunk b, half h, this lane's column (row 0 when absent)
+    // row byte offsets (the step's n goes into the scalar offset); absent
+    // rows read as zeros
+    const Rsrc r_g = make_rsrc(p.G, p.g_bytes);
+    const Rsrc r_x = make_rsrc(p.X, p.x_bytes);
+    const uint32_t q8 = 8u * (uint32_t)(lane >> 4);
+    uint32_t xoff[CB][2];
 #pragma unroll
     for (int b = 0; b < CB; b++)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int col = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + 16 * h + (lane & 15)] : -1;
-            xoff[b][h] = (long)(col >= 0 ? col : 0) * p.ldx;
+            xoff[b][h] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + q8) * XB::BES : kOOB;
         }
-    long goff[PPG];
+    uint32_t goff[PPG];
 #pragma unroll
     for (int i = 0; i < PPG; i++) {
         const int row = g * p.rpw + 16 * (p0 + i) + (lane & 15);
-        goff[i] = (long)(row < p.n_rows ? row : 0) * p.ldg;
+        goff[i] = row < p.n_rows ? ((uint32_t)row * (uint32_t)p.ldg + q8) * GB::BES : kOOB;
     }
-    const int q8 = 8 * (lane >> 4);
     auto load = [&](int n, Step &st) {
 #pragma unroll
-        for (int i = 0; i < PPG; i++) st.g[i] = GR::load(p.G, goff[i] + n + q8);
+        for (int i = 0; i < PPG; i++) st.g[i] = GB::load(r_g, goff[i], (uint32_t)n * GB::BES);
 #pragma unroll
         for (int b = 0; b < CB; b++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) st.x[b][h] = XR::load(p.X, xoff[b][h] + n + q8);
+            for (int h = 0; h < 2; h++) st.x[b][h] = XB::load(r_x, xoff[b][h], (uint32_t)n * XB::BES);
     };
     auto comp = [&](const Step &st) {
         Oct ga[PPG];
@@ -902,17 +1033,30 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
             }
         }
     };
-    const int n_last = p.N - 32;
+    const int n_last = n_hi - 32;
     Step st[2];
-    load(0, st[0]);
-    load(min(32, n_last), st[1]);
-    for (int n = 0; n < p.N; n += 64) {
+    load(n_lo, st[0]);
+    load(min(n_lo + 32, n_last), st[1]);
+    for (int n = n_lo; n < n_hi; n += 64) {
         comp(st[0]);
         load(min(n + 64, n_last), st[0]);
-        if (n + 32 < p.N) comp(st[1]);
+        if (n + 32 < n_hi) comp(st[1]);
         load(min(n + 96, n_last), st[1]);
     }
-    // sample: stage each (chunk, panel) 16x32 tile in LDS, then read entry slots
+}
+
+// One wave = (group g, chunks [c0, c0+CB), panels from p0) over all of N; each
+// (chunk, panel) 16x32 tile is staged in LDS (`tile`, 16*33 floats) and
+// sampled at the entry slots, giving dW in tiled order.
+template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
+__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, int p0, float *tile)
+{
+    constexpr int PPG = RPW / 16;
+    const int lane = lane_id();
+    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
+    const int plan_ppg = p.rpw >> 4;
+    f32x4 acc[CB][2][PPG];
+    sddmm_acc<RPW, CB, AT, GT>(p, g, c0, p0, nc, 0, p.N, acc);
 #pragma unroll
     for (int b = 0; b < CB; b++) {
         if (b >= nc) break;
@@ -934,12 +1078,57 @@ __device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, 
     }
 }
 
+// Standalone SDDMM: workgroup = one unit; its WS = blockDim.x/64 waves split the
+// contraction over N into WS contiguous ranges, and each sampled entry sums the
+// WS partial tiles in wave order (deterministic).
+constexpr int kSddmmMaxWaves = 4;
+
 template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__global__ __launch_bounds__(64) void sddmm_kernel(SddmmParams p)
+__global__ __launch_bounds__(64 * kSddmmMaxWaves) void sddmm_kernel(SddmmParams p)
 {
-    __shared__ float tile[16 * 33];
-    sddmm_unit<RPW, CB, AT, GT>(p, p.unit_group[blockIdx.x], p.unit_chunk[blockIdx.x],
-                                p.unit_panel ? p.unit_panel[blockIdx.x] : 0, tile);
+    constexpr int PPG = RPW / 16;
+    __shared__ float red[kSddmmMaxWaves][16 * 33];
+    const int WS = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+    const int g = p.unit_group[blockIdx.x], c0 = p.unit_chunk[blockIdx.x];
+    const int p0 = p.unit_panel ? p.unit_panel[blockIdx.x] : 0;
+    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
+    const int plan_ppg = p.rpw >> 4;
+    const int steps = p.N >> 5;
+    const int s_lo = steps * w / WS, s_hi = steps * (w + 1) / WS;
+    f32x4 acc[CB][2][PPG];
+    if (s_lo < s_hi) {
+        sddmm_acc<RPW, CB, AT, GT>(p, g, c0, p0, nc, 32 * s_lo, 32 * s_hi, acc);
+    } else {
+#pragma unroll
+        for (int b = 0; b < CB; b++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    float *tile = red[w];
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+        if (b >= nc) break;
+#pragma unroll
+        for (int i = 0; i < PPG; i++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][i][j];
+            __syncthreads();
+            const int c = c0 + b;
+            const int e0 = p.seg_ptr[c * plan_ppg + p0 + i], e1 = p.seg_ptr[c * plan_ppg + p0 + i + 1];
+            for (int e = e0 + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
+                const int slot = p.entry_slot[e];
+                const int at = (slot >> 5) * 33 + (slot & 31);
+                float v = red[0][at];
+                for (int q = 1; q < WS; q++) v += red[q][at];
+                p.out[e] = v;
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,6 +1263,7 @@ __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
     p.chunk_cols = d.chunk_cols;
     p.seg_ptr = d.seg_ptr;
     p.entry_slot = d.entry_slot;
+    p.entry_lds = d.entry_lds;
     p.entry_src = d.entry_src;
     p.work_order = nullptr;
     p.a = d.a;
@@ -1093,6 +1283,8 @@ __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
     p.rpw = d.rpw;
     p.n_units = 0;
     p.xcd_share = -1;
+    p.b_bytes = d.b_bytes;
+    p.n_chunks = d.n_chunks;
     return p;
 }
 
@@ -1115,6 +1307,8 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     p.unit_group = nullptr;
     p.unit_chunk = nullptr;
     p.unit_panel = nullptr;
+    p.g_bytes = d.a_bytes;
+    p.x_bytes = d.b_bytes;
     return p;
 }
 
@@ -1140,7 +1334,7 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
 __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                    const BatchUnitDev *__restrict__ units, int n_units, int *counters)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[64 * 32 * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[32 * tr_pitch<64>() * 2];
     __shared__ __attribute__((aligned(16))) uint8_t lds_a[512 * 2 > 16 * 33 * 4 ? 512 * 2 : 16 * 33 * 4];
     const int lane = threadIdx.x;
     for (;;) {
@@ -1230,6 +1424,13 @@ tca_status launch_warm_up(hipStream_t stream)
     return TCA_OK;
 }
 
+uint32_t operand_bytes(long rows, long N, long ld, int elem_bytes)
+{
+    if (rows <= 0 || N <= 0) return 16;   // never read; a non-empty range keeps the descriptor valid
+    const long b = ((rows - 1) * ld + N) * (long)elem_bytes;
+    return b < (1L << 31) - 64 ? (uint32_t)b : 0u;
+}
+
 int default_tile_n(const tca_plan *p, int N)
 {
     if (p->host.rpw >= 128) return N >= 256 ? 128 : 64;
@@ -1249,7 +1450,8 @@ static constexpr long kWideTargetGroups = 1024;
 
 static int spmm_slot_bytes(int rpw, int nt, int es)
 {
-    const int tiles = nt * 32 * es + (rpw / 16) * 512 * es;
+    const int btile = es == 2 ? 32 * (nt == 16 ? 48 : nt + 16) * 2 : nt * 32 * es;   // = SpmmLds<...>::B
+    const int tiles = btile + (rpw / 16) * 512 * es;
     const int red = rpw * (nt + 4) * 4;
     return tiles > red ? tiles : red;
 }
@@ -1304,6 +1506,7 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.chunk_cols = p->d_chunk_cols;
     prm.seg_ptr = p->d_seg_ptr;
     prm.entry_slot = p->d_entry_slot;
+    prm.entry_lds = p->d_entry_lds;
     prm.entry_src = p->d_entry_src;
     prm.work_order = p->d_work_order;
     prm.a = a;
@@ -1319,7 +1522,10 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     prm.n_groups = t.n_groups;
     prm.rpw = t.rpw;
     prm.nnz = t.nnz;
-    if (prm.n_tiles > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: too many N tiles");
+    prm.n_chunks = t.n_chunks;
+    prm.b_bytes = operand_bytes(t.n_cols, N, ldb, b_dtype == TCA_F32 ? 4 : b_dtype == TCA_E4M3 ? 1 : 2);
+    if (prm.b_bytes == 0 || (long)t.nnz * 4 >= (1L << 31) || (long)t.n_chunks * 128 >= (1L << 31))
+        return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: B or the plan's tables reach 2 GiB (32-bit offsets)");
 
     const long base = (long)t.n_groups * prm.n_tiles;
     const int maxc = t.max_group_chunks;
@@ -1443,7 +1649,15 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     prm.unit_group = static_cast<const int32_t *>(mp->d_sddmm_units);
     prm.unit_chunk = static_cast<const int32_t *>(mp->d_sddmm_units) + mp->n_sddmm_units;
     prm.unit_panel = static_cast<const int32_t *>(mp->d_sddmm_units) + 2 * mp->n_sddmm_units;
-    hipLaunchKernelGGL(fn, dim3(mp->n_sddmm_units), dim3(64), 0, stream, prm);
+    prm.g_bytes = operand_bytes(t.n_rows, N, ldg, g_dtype == TCA_F32 ? 4 : 2);
+    prm.x_bytes = operand_bytes(t.n_cols, N, ldx, 2);
+    if (prm.g_bytes == 0 || prm.x_bytes == 0)
+        return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: G or X reaches 2 GiB (32-bit offsets)");
+    // split the contraction over N across waves while the grid is small
+    const int steps = N / 32;
+    int WS = 1;
+    while (WS < kSddmmMaxWaves && (long)mp->n_sddmm_units * WS < 4096 && steps / (2 * WS) >= 4) WS *= 2;
+    hipLaunchKernelGGL(fn, dim3(mp->n_sddmm_units), dim3(64 * WS), 0, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }

@@ -1,9 +1,10 @@
 """Batched persistent launch over the twelve uneven layers (forward SpMM,
 transposed SpMM dX and SDDMM dW of every layer in ONE kernel): the outputs
-must match the per-call kernels (bit for bit where neither splits a group's
-chunks; to FP32 summation-order noise where the per-call launch splits and
-accumulates), match the FP64 oracle, and stay bit-identical across repeated
-HIP-graph replays (the launch resets its own work counter)."""
+must match the per-call kernels to FP32 summation-order noise (the per-call
+launches split a group's chunks, or the SDDMM contraction, across waves and
+add the partial sums in a fixed order), match the FP64 oracle, and stay
+bit-identical across repeated HIP-graph replays (the launch resets its own
+work counter)."""
 import numpy as np
 import pytest
 import torch
@@ -52,8 +53,7 @@ def test_batch_equals_per_call_and_oracle(cuda):
         Y = T.spmm(L["plan"], L["w"], L["X"], tile_n=L["tile"])
         dX = T.spmm(L["plan_t"], L["w"], L["G"], tile_n=L["tile_t"])
         dW = T.sddmm(L["plan"], L["G"], L["X"])
-        assert torch.equal(dW, L["dW"])
-        for got, ref in ((L["Y"], Y), (L["dX"], dX)):
+        for got, ref in ((L["Y"], Y), (L["dX"], dX), (L["dW"], dW)):
             assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
     for L in layers[::5]:
         W, X, G = L["case"].A, L["case"].B, L["case"].extra["G"]

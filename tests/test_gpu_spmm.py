@@ -210,6 +210,7 @@ def test_sddmm_dw(cuda):
         plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
         for Gd in (dev(G), dev_f16(G.astype(np.float16).view(np.uint16))):
             out = T.sddmm(plan, Gd, dev_f16(X))
+            assert torch.equal(out, T.sddmm(plan, Gd, dev_f16(X)))     # fixed summation order
             csr = plan.unpack(out).cpu().numpy()
             assert rel_l2(csr, ref_q) <= 1e-5
             assert rel_l2(csr, ref) <= 1e-3
