@@ -84,19 +84,34 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) +
-                                                     " (batched SDDMM: rows_per_wave 16, f16 X, f32 or f16 G)");
+                                                     " (batched SDDMM: f16 X, f32 or f16 G)");
+            // the SDDMM runs on the plan's 64-row tiling and writes through dst
+            const tca_plan *aux = nullptr;
+            const int32_t *dst = nullptr;
+            tca_status s = tca::sddmm_tiling(p, &aux, &dst);
+            if (s != TCA_OK) return s;
+            const tca::PlanTables &ta = aux->host;
+            d.grp_chunk_ptr = aux->d_grp_chunk_ptr;
+            d.chunk_cols = aux->d_chunk_cols;
+            d.seg_ptr = aux->d_seg_ptr;
+            d.entry_slot = aux->d_entry_slot;
+            d.entry_lds = aux->d_entry_lds;
+            d.entry_src = aux->d_entry_src;
+            d.dst = dst;
+            d.rpw = ta.rpw;
+            d.n_chunks = ta.n_chunks;
             d.ldb = it.ldb;   // X
             d.ldc = it.lda;   // G
             d.a_bytes = tca::operand_bytes(t.n_rows, it.N, it.lda, it.a_dtype == TCA_F32 ? 4 : 2);
             d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, 2);
             if (d.a_bytes == 0 || d.b_bytes == 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) + " reaches 2 GiB");
-            for (int gi = 0; gi < t.n_groups; gi++) {
-                const int g = t.work_order[gi];
-                for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {
-                    const int nc = std::min(tca::kBatchSddmmCB, t.grp_chunk_ptr[g + 1] - c);
+            for (int gi = 0; gi < ta.n_groups; gi++) {
+                const int g = ta.work_order[gi];
+                for (int c = ta.grp_chunk_ptr[g]; c < ta.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {
+                    const int nc = std::min(tca::kBatchSddmmCB, ta.grp_chunk_ptr[g + 1] - c);
                     units.push_back({i, g, c, 0, 0, 0, 0, 0});
-                    cost.push_back(nc * 2.0 * ppg * (it.N / 32.0) + 4.0);
+                    cost.push_back(nc * 2.0 * (tca::kSddmmRows / 16) * (it.N / 32.0) / 4.0 + 4.0);
                 }
             }
         } else {
@@ -104,7 +119,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         }
     }
     // Split SpMM units much longer than the average share of a workgroup into
-    // chunk-range pieces (>= 4 chunks); their items accumulate into a zeroed output.
+    // chunk-range pieces (>= 16 chunks, 4 per wave of the workgroup); their
+    // items accumulate into a zeroed output.
     int dev_id = 0, cus = 0;
     TCA_HIP_TRY(hipGetDevice(&dev_id));
     TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
@@ -121,7 +137,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             const bool is_spmm = items[un.item].op == TCA_OP_SPMM;
             const int nch = un.c_hi - un.c_lo;
             int P = 1;
-            if (is_spmm && cost[k] > 2.0 * share) P = std::min((int)(cost[k] / share), nch / 4);
+            if (is_spmm && cost[k] > 2.0 * share) P = std::min((int)(cost[k] / share), nch / 16);
             if (P <= 1) {
                 u2.push_back(un);
                 c2.push_back(cost[k]);

@@ -159,18 +159,93 @@ extern "C" tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_pe
 extern "C" tca_status tca_plan_destroy(tca_plan *plan)
 {
     if (!plan) return TCA_OK;
+    if (plan->sddmm_aux) tca_plan_destroy(plan->sddmm_aux);
     int cur = 0;
     bool switched = false;
-    if ((plan->d_base || plan->d_sddmm_units || !plan->unit_tables.empty()) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
+    if ((plan->d_base || plan->d_sddmm_dst || plan->d_sddmm_units[0] || plan->d_sddmm_units[1] ||
+         !plan->unit_tables.empty()) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
         switched = hipSetDevice(plan->device) == hipSuccess;
     if (plan->d_base) (void)hipFree(plan->d_base);
-    if (plan->d_sddmm_units) (void)hipFree(plan->d_sddmm_units);
+    if (plan->d_sddmm_dst) (void)hipFree(plan->d_sddmm_dst);
+    for (void *u : plan->d_sddmm_units)
+        if (u) (void)hipFree(u);
     for (auto &u : plan->unit_tables)
         if (u.dev) (void)hipFree(u.dev);
     if (switched) (void)hipSetDevice(cur);
     delete plan;
     return TCA_OK;
 }
+
+namespace tca {
+
+tca_status sddmm_tiling(const tca_plan *cp, const tca_plan **aux, const int32_t **dst)
+{
+    tca_plan *p = const_cast<tca_plan *>(cp);
+    if (p->host.rpw == kSddmmRows) {
+        *aux = p;
+        *dst = nullptr;
+        return TCA_OK;
+    }
+    if (!p->sddmm_aux) {
+        const PlanTables &t = p->host;
+        tca_plan *a = new (std::nothrow) tca_plan();
+        if (!a) return fail(TCA_ERR_ALLOC, "sddmm: host allocation");
+        tca_status s = build_tables(t.n_rows, t.n_cols, t.nnz, p->csr_row_offset.data(), p->csr_col_idx.data(),
+                                    kSddmmRows, a->host);
+        if (s == TCA_OK) s = upload(a);
+        if (s != TCA_OK) {
+            tca_plan_destroy(a);
+            return s;
+        }
+        // entry of the 64-row tiling -> CSR index -> this plan's tiled index
+        std::vector<int32_t> tiled_of_csr(t.nnz), d(t.nnz);
+        for (int e = 0; e < t.nnz; e++) tiled_of_csr[t.entry_src[e]] = e;
+        for (int e = 0; e < t.nnz; e++) d[e] = tiled_of_csr[a->host.entry_src[e]];
+        if (t.nnz > 0) {
+            if (hipMalloc(&p->d_sddmm_dst, sizeof(int32_t) * t.nnz) != hipSuccess) {
+                tca_plan_destroy(a);
+                return fail(TCA_ERR_ALLOC, "sddmm: device allocation");
+            }
+            if (hipMemcpy(p->d_sddmm_dst, d.data(), sizeof(int32_t) * t.nnz, hipMemcpyHostToDevice) != hipSuccess) {
+                tca_plan_destroy(a);
+                return fail(TCA_ERR_HIP, "sddmm: upload");
+            }
+        }
+        p->sddmm_aux = a;
+    }
+    *aux = p->sddmm_aux;
+    *dst = p->d_sddmm_dst;
+    return TCA_OK;
+}
+
+tca_status sddmm_units(const tca_plan *cp, int cb, const int32_t **units, int *n)
+{
+    tca_plan *p = const_cast<tca_plan *>(cp);
+    const int slot = cb == 4 ? 0 : 1;
+    if (!p->d_sddmm_units[slot]) {
+        const PlanTables &t = p->host;
+        std::vector<int32_t> ug, uc;
+        for (int gi = 0; gi < t.n_groups; gi++) {
+            const int g = t.work_order[gi];
+            for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += cb) {
+                ug.push_back(g);
+                uc.push_back(c);
+            }
+        }
+        const size_t m = ug.size();
+        ug.insert(ug.end(), uc.begin(), uc.end());
+        if (hipMalloc(&p->d_sddmm_units[slot], sizeof(int32_t) * std::max<size_t>(ug.size(), 2)) != hipSuccess)
+            return fail(TCA_ERR_ALLOC, "sddmm: unit table allocation");
+        if (!ug.empty())
+            TCA_HIP_TRY(hipMemcpy(p->d_sddmm_units[slot], ug.data(), sizeof(int32_t) * ug.size(), hipMemcpyHostToDevice));
+        p->n_sddmm_units[slot] = (int)m;
+    }
+    *units = static_cast<const int32_t *>(p->d_sddmm_units[slot]);
+    *n = p->n_sddmm_units[slot];
+    return TCA_OK;
+}
+
+}  // namespace tca
 
 extern "C" tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *info)
 {

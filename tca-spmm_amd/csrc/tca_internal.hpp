@@ -57,9 +57,15 @@ struct tca_plan {
     const uint16_t *d_entry_lds = nullptr;   // entry_slot as an index into the SpMM LDS A tile
     const int32_t *d_entry_src = nullptr;
     const int32_t *d_work_order = nullptr;
-    // SDDMM work units (group, first chunk), built on first use
-    void *d_sddmm_units = nullptr;
-    int n_sddmm_units = 0;
+    // SDDMM: the 64-row tiling its kernel runs on (this plan itself when it has
+    // 64-row groups) and, for each of that tiling's entries, its index in this
+    // plan's tiled order (null for this plan itself); built on first use
+    tca_plan *sddmm_aux = nullptr;
+    int32_t *d_sddmm_dst = nullptr;
+    // SDDMM work units (group, first chunk) of this plan as an SDDMM tiling,
+    // for 4- and 2-chunk blocks, built on first use
+    void *d_sddmm_units[2] = {nullptr, nullptr};
+    int n_sddmm_units[2] = {0, 0};
     // SpMM unit tables (group, pieces, c_lo, c_hi), one per split factor
     struct UnitTable {
         int pieces = 1, count = 0;
@@ -87,6 +93,13 @@ tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_
                                 hipStream_t stream);
 int default_tile_n(const tca_plan *p, int N);
 tca_status launch_warm_up(hipStream_t stream);
+// The SDDMM tiling of a forward plan (tca_capi.cpp): *aux has 64-row groups,
+// *dst maps its entries to p's tiled order (null when *aux == p).
+constexpr int kSddmmRows = 64;
+tca_status sddmm_tiling(const tca_plan *p, const tca_plan **aux, const int32_t **dst);
+// SDDMM work units (group, first chunk) of a 64-row tiling for blocks of cb (4
+// or 2) chunks, heavy groups first: n pairs at *units (groups, then chunks).
+tca_status sddmm_units(const tca_plan *aux, int cb, const int32_t **units, int *n);
 // Bytes from the first to the last element of a rows x N row-major operand with
 // leading dimension ld; 0 when that is not below 2^31 (the kernels address
 // operands with 32-bit byte offsets).
@@ -97,9 +110,10 @@ struct BatchItemDev {
     int32_t variant;   // kernel variant id (batch_variant)
     int32_t n_rows, N, ldb, ldc, n_tiles, rpw, nnz, n_chunks;
     uint32_t a_bytes, b_bytes;   // SPMM: -, B; SDDMM: G, X (bytes from first to last used element)
-    const int32_t *grp_chunk_ptr, *chunk_cols, *seg_ptr;
+    const int32_t *grp_chunk_ptr, *chunk_cols, *seg_ptr;   // SDDMM: of the 64-row tiling
     const uint16_t *entry_slot, *entry_lds;
     const int32_t *entry_src;
+    const int32_t *dst;          // SDDMM: output index of each entry (null: identity)
     const void *a, *b;
     float *out;
 };
@@ -112,7 +126,7 @@ struct ZeroDesc {              // an output that split units accumulate into
     float *C;
     int32_t rows, N, ldc, pad;
 };
-constexpr int kBatchSddmmCB = 4;
+constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed);
 // resident one-wave workgroups of batch_kernel per compute unit

@@ -568,24 +568,20 @@ struct SpmmLds {
     static constexpr int WMAX = 65536 / SLOT >= 8 ? 8 : (65536 / SLOT >= 1 ? 65536 / SLOT : 1);
 };
 
-// Standalone launch: workgroup -> (unit of the plan's unit table, N-tile) by
-// xcd_work.  Its W = blockDim.x/64 waves split the unit's chunk range
-// evenly, each into its own LDS slot; the W partial tiles are then summed in
-// wave order (deterministic) and stored as float4 rows.  Units of a group
-// split across workgroups (p.split) accumulate atomically into a zeroed C.
+// One workgroup of W = blockDim.x/64 waves on one (group g, N-tile, chunk
+// range [c_lo, c_hi)): the waves split the range evenly, each into its own LDS
+// slot of `slot` bytes; the W partial tiles are then summed in wave order
+// (deterministic) and stored as float4 rows, or added atomically (`atomic`:
+// other workgroups hold other chunk ranges of the group; C was zeroed).
 template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
-__global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel(SpmmParams p)
+__device__ __forceinline__ void spmm_wg_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, bool atomic,
+                                             uint8_t *smem, int slot)
 {
     using L = SpmmLds<RPW, NT, AT>;
-    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
-    const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint8_t *mine = smem + w * L::SLOT;
-    int ui, tile;
-    if (!xcd_work(p, blockIdx.x, ui, tile)) return;
-    const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
-    const int g = u.x;
-    const int nch = u.w - u.z;
-    const int lo = u.z + (nch * w) / W, hi = u.z + (nch * (w + 1)) / W;
+    const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+    uint8_t *mine = smem + w * slot;
+    const int nch = c_hi - c_lo;
+    const int lo = c_lo + (nch * w) / W, hi = c_lo + (nch * (w + 1)) / W;
     f32x4 acc[RPW / 16][NT / 16];
     spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, g, tile, lo, hi, mine, mine + L::B, acc);
 
@@ -602,7 +598,7 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
         const int r = idx / NT, c = idx - r * NT;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int q = 0; q < W; q++) {
-            const float4 t = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(smem + q * L::SLOT) +
+            const float4 t = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(smem + q * slot) +
                                                                r * L::RED_LD + c);
             v.x += t.x;
             v.y += t.y;
@@ -623,7 +619,7 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
             v.w *= p.col_scale[n + 3];
         }
         float *dst = p.C + (long)row * p.ldc + n;
-        if (p.split) {
+        if (atomic) {
             unsafeAtomicAdd(dst, v.x);
             unsafeAtomicAdd(dst + 1, v.y);
             unsafeAtomicAdd(dst + 2, v.z);
@@ -632,6 +628,20 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
             *reinterpret_cast<float4 *>(dst) = v;
         }
     }
+}
+
+// Standalone launch: workgroup -> (unit of the plan's unit table, N-tile) by
+// xcd_work, run by spmm_wg_unit.  Units of a group split across workgroups
+// (p.split) accumulate atomically into a zeroed C.
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+__global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel(SpmmParams p)
+{
+    using L = SpmmLds<RPW, NT, AT>;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    int ui, tile;
+    if (!xcd_work(p, blockIdx.x, ui, tile)) return;
+    const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
+    spmm_wg_unit<RPW, NT, AT, BT, INDIRECT>(p, u.x, tile, u.z, u.w, p.split != 0, smem, L::SLOT);
 }
 
 // ---------------------------------------------------------------------------
@@ -956,179 +966,134 @@ struct SddmmParams {
     int n_rows, N, ldg, ldx, rpw, n_blocks_per_group_max;
     const int32_t *unit_group;   // per workgroup: group id
     const int32_t *unit_chunk;   // per workgroup: first chunk
-    const int32_t *unit_panel;   // per workgroup: first panel (plans wider than the kernel), or null
     uint32_t g_bytes, x_bytes;   // bytes of G and X from the first to the last used element
+    const int32_t *dst;          // out index of each entry (null: the entry's own index)
 };
 
-// One wave accumulates the 16x32 result tiles of (group g, chunks
-// [c0, c0+nc), panels [p0, p0+RPW/16) of the plan's rpw/16) over the
-// contraction range [n_lo, n_hi) (multiples of 32, n_hi > n_lo).  The
-// contraction runs as a two-stage register pipeline: the G and X octets of
-// step n+32 are loaded (unconditionally, with row addresses clamped into
-// range) while step n is multiplied.  Rows past the matrix and padded columns
-// only feed accumulator cells no entry samples.
-template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__device__ __forceinline__ void sddmm_acc(const SddmmParams &p, int g, int c0, int p0, int nc, int n_lo, int n_hi,
-                                          f32x4 (&acc)[CB][2][RPW / 16])
+// SDDMM with shared X rows.  The plan here has 64-row groups: a workgroup of 4
+// waves owns (group g, chunks [c0, c0+CB)), wave w the group's panel w.  Per
+// contraction step (32 n) the workgroup loads the gathered X rows of its
+// chunks once -- CB*32 rows x 32 n, 16-byte pieces, double-buffered in LDS with
+// one barrier per step -- and every wave reads its MFMA B operands (8
+// consecutive n of one X row) from there; each wave's A operands are 8
+// consecutive n of its own G rows, straight from memory, one step ahead.  X is
+// fetched once per 64 rows instead of once per 16.  Each wave then stages its
+// 16x32 result tiles in its own LDS area and samples them at the entry slots,
+// writing out[dst[e]] (dW in the caller's plan's tiled order).
+constexpr int kShRows = 64;           // rows per group of the SDDMM tiling
+constexpr int kShPitch = 40;          // X tile row pitch (elements): 32 n + 8
+
+template <int CB>
+struct ShLds {
+    static constexpr int XT = CB * 32 * kShPitch * 2;           // one X tile (bytes)
+    static constexpr int STAGE = 16 * 33 * 4;                   // one wave's result tile
+    static constexpr int BYTES = 2 * XT + 4 * STAGE;
+    static constexpr int XP = CB * 32 * 4 / 256;                // X pieces per thread and step
+};
+
+template <int CB, tca_dtype AT, tca_dtype GT>
+__device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, int c0, uint8_t *smem)
 {
-    constexpr int PPG = RPW / 16;
+    using L = ShLds<CB>;
     using Oct = typename Elt<AT>::Oct;
     using GR = RowLoad<AT, GT>;
-    using XR = RowLoad<AT, AT>;
     using GB = RowBuf<AT, GT>;
     using XB = RowBuf<AT, AT>;
-    struct Step {
-        typename GB::Raw g[PPG];
-        typename XB::Raw x[CB][2];
-    };
+    const int w = threadIdx.x >> 6, lane = lane_id(), tid = w * 64 + lane;
+    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
+    const Rsrc r_g = make_rsrc(p.G, p.g_bytes);
+    const Rsrc r_x = make_rsrc(p.X, p.x_bytes);
 
-    const int lane = lane_id();
+    // this thread's X pieces: tile row (chunk b = row/32), 8-n octet q
+    uint32_t xsrc[L::XP];
+    int xdst[L::XP];
+#pragma unroll
+    for (int j = 0; j < L::XP; j++) {
+        const int pc = tid + 256 * j, row = pc >> 2, q = pc & 3, b = row >> 5;
+        const int col = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + (row & 31)] : -1;
+        xsrc[j] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + 8u * q) * 2u : kOOB;
+        xdst[j] = (row * kShPitch + 8 * q) * 2;
+    }
+    const int grow = g * kShRows + 16 * w + (lane & 15);
+    const uint32_t gsrc = grow < p.n_rows ? ((uint32_t)grow * (uint32_t)p.ldg + 8u * (lane >> 4)) * GB::BES : kOOB;
+    // B operand of (chunk b, half h): X tile row 32b + 16h + lane%16, octet lane/16
+    const int xread = ((lane & 15) * kShPitch + 8 * (lane >> 4)) * 2;
+
+    f32x4 acc[CB][2];
 #pragma unroll
     for (int b = 0; b < CB; b++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) acc[b][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int steps = p.N >> 5;
+    typename XB::Raw xr[L::XP];
+    typename GB::Raw gr[2];
+    auto load_x = [&](int s) {
+#pragma unroll
+        for (int j = 0; j < L::XP; j++) xr[j] = XB::load(r_x, xsrc[j], (uint32_t)(32 * s) * 2u);
+    };
+    auto put_x = [&](int s) {
+        uint8_t *buf = smem + (s & 1) * L::XT;
+#pragma unroll
+        for (int j = 0; j < L::XP; j++) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[j];
+    };
+    // prologue: X of step 0 in LDS, X of step 1 and G of steps 0, 1 in registers
+    load_x(0);
+    gr[0] = GB::load(r_g, gsrc, 0);
+    gr[1] = GB::load(r_g, gsrc, (uint32_t)(32 * min(1, steps - 1)) * GB::BES);
+    put_x(0);
+    load_x(min(1, steps - 1));
+    __syncthreads();
+    for (int s0 = 0; s0 < steps; s0 += 2) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int s = s0 + k;
+            if (s < steps) {
+                const uint8_t *buf = smem + (s & 1) * L::XT;
+                const Oct ga = GR::cvt(gr[k]);
+#pragma unroll
+                for (int b = 0; b < CB; b++) {
+                    if (b < nc) {
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            const Oct xb = *reinterpret_cast<const Oct *>(buf + xread + (32 * b + 16 * h) * kShPitch * 2);
+                            acc[b][h] = Elt<AT>::mma(ga, xb, acc[b][h]);
+                        }
+                    }
+                }
+                gr[k] = GB::load(r_g, gsrc, (uint32_t)(32 * min(s + 2, steps - 1)) * GB::BES);
+                put_x(s + 1);                       // X of step s+1 (a clamped re-read past the end)
+                load_x(min(s + 2, steps - 1));
+                __syncthreads();
+            }
+        }
+    }
+    // sample this wave's panel: stage each chunk's 16x32 tile in the wave's area
+    float *tile = reinterpret_cast<float *>(smem + 2 * L::XT + w * L::STAGE);
+    const int plan_ppg = p.rpw >> 4;
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+        if (b >= nc) break;
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
-            for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    // row byte offsets (the step's n goes into the scalar offset); absent
-    // rows read as zeros
-    const Rsrc r_g = make_rsrc(p.G, p.g_bytes);
-    const Rsrc r_x = make_rsrc(p.X, p.x_bytes);
-    const uint32_t q8 = 8u * (uint32_t)(lane >> 4);
-    uint32_t xoff[CB][2];
-#pragma unroll
-    for (int b = 0; b < CB; b++)
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int col = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + 16 * h + (lane & 15)] : -1;
-            xoff[b][h] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + q8) * XB::BES : kOOB;
+            for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][j];
+        wave_lds_sync();
+        const int c = c0 + b;
+        const int e0 = p.seg_ptr[c * plan_ppg + w], e1 = p.seg_ptr[c * plan_ppg + w + 1];
+        for (int e = e0 + lane; e < e1; e += 64) {
+            const int slot = p.entry_slot[e];
+            p.out[p.dst ? p.dst[e] : e] = tile[(slot >> 5) * 33 + (slot & 31)];
         }
-    uint32_t goff[PPG];
-#pragma unroll
-    for (int i = 0; i < PPG; i++) {
-        const int row = g * p.rpw + 16 * (p0 + i) + (lane & 15);
-        goff[i] = row < p.n_rows ? ((uint32_t)row * (uint32_t)p.ldg + q8) * GB::BES : kOOB;
-    }
-    auto load = [&](int n, Step &st) {
-#pragma unroll
-        for (int i = 0; i < PPG; i++) st.g[i] = GB::load(r_g, goff[i], (uint32_t)n * GB::BES);
-#pragma unroll
-        for (int b = 0; b < CB; b++)
-#pragma unroll
-            for (int h = 0; h < 2; h++) st.x[b][h] = XB::load(r_x, xoff[b][h], (uint32_t)n * XB::BES);
-    };
-    auto comp = [&](const Step &st) {
-        Oct ga[PPG];
-#pragma unroll
-        for (int i = 0; i < PPG; i++) ga[i] = GR::cvt(st.g[i]);
-#pragma unroll
-        for (int b = 0; b < CB; b++) {
-            if (b < nc) {
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const Oct xb = XR::cvt(st.x[b][h]);
-#pragma unroll
-                    for (int i = 0; i < PPG; i++) acc[b][h][i] = Elt<AT>::mma(ga[i], xb, acc[b][h][i]);
-                }
-            }
-        }
-    };
-    const int n_last = n_hi - 32;
-    Step st[2];
-    load(n_lo, st[0]);
-    load(min(n_lo + 32, n_last), st[1]);
-    for (int n = n_lo; n < n_hi; n += 64) {
-        comp(st[0]);
-        load(min(n + 64, n_last), st[0]);
-        if (n + 32 < n_hi) comp(st[1]);
-        load(min(n + 96, n_last), st[1]);
+        wave_lds_sync();
     }
 }
 
-// One wave = (group g, chunks [c0, c0+CB), panels from p0) over all of N; each
-// (chunk, panel) 16x32 tile is staged in LDS (`tile`, 16*33 floats) and
-// sampled at the entry slots, giving dW in tiled order.
-template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__device__ __forceinline__ void sddmm_unit(const SddmmParams &p, int g, int c0, int p0, float *tile)
+template <int CB, tca_dtype AT, tca_dtype GT>
+__global__ __launch_bounds__(256) void sddmm_shared_kernel(SddmmParams p)
 {
-    constexpr int PPG = RPW / 16;
-    const int lane = lane_id();
-    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
-    const int plan_ppg = p.rpw >> 4;
-    f32x4 acc[CB][2][PPG];
-    sddmm_acc<RPW, CB, AT, GT>(p, g, c0, p0, nc, 0, p.N, acc);
-#pragma unroll
-    for (int b = 0; b < CB; b++) {
-        if (b >= nc) break;
-#pragma unroll
-        for (int i = 0; i < PPG; i++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][i][j];
-            __syncthreads();
-            const int c = c0 + b;
-            const int e0 = p.seg_ptr[c * plan_ppg + p0 + i], e1 = p.seg_ptr[c * plan_ppg + p0 + i + 1];
-            for (int e = e0 + lane; e < e1; e += 64) {
-                const int slot = p.entry_slot[e];
-                p.out[e] = tile[(slot >> 5) * 33 + (slot & 31)];
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Standalone SDDMM: workgroup = one unit; its WS = blockDim.x/64 waves split the
-// contraction over N into WS contiguous ranges, and each sampled entry sums the
-// WS partial tiles in wave order (deterministic).
-constexpr int kSddmmMaxWaves = 4;
-
-template <int RPW, int CB, tca_dtype AT, tca_dtype GT>
-__global__ __launch_bounds__(64 * kSddmmMaxWaves) void sddmm_kernel(SddmmParams p)
-{
-    constexpr int PPG = RPW / 16;
-    __shared__ float red[kSddmmMaxWaves][16 * 33];
-    const int WS = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
-    const int g = p.unit_group[blockIdx.x], c0 = p.unit_chunk[blockIdx.x];
-    const int p0 = p.unit_panel ? p.unit_panel[blockIdx.x] : 0;
-    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
-    const int plan_ppg = p.rpw >> 4;
-    const int steps = p.N >> 5;
-    const int s_lo = steps * w / WS, s_hi = steps * (w + 1) / WS;
-    f32x4 acc[CB][2][PPG];
-    if (s_lo < s_hi) {
-        sddmm_acc<RPW, CB, AT, GT>(p, g, c0, p0, nc, 32 * s_lo, 32 * s_hi, acc);
-    } else {
-#pragma unroll
-        for (int b = 0; b < CB; b++)
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int i = 0; i < PPG; i++) acc[b][h][i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-    float *tile = red[w];
-#pragma unroll
-    for (int b = 0; b < CB; b++) {
-        if (b >= nc) break;
-#pragma unroll
-        for (int i = 0; i < PPG; i++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][i][j];
-            __syncthreads();
-            const int c = c0 + b;
-            const int e0 = p.seg_ptr[c * plan_ppg + p0 + i], e1 = p.seg_ptr[c * plan_ppg + p0 + i + 1];
-            for (int e = e0 + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
-                const int slot = p.entry_slot[e];
-                const int at = (slot >> 5) * 33 + (slot & 31);
-                float v = red[0][at];
-                for (int q = 1; q < WS; q++) v += red[q][at];
-                p.out[e] = v;
-            }
-            __syncthreads();
-        }
-    }
+    __shared__ __attribute__((aligned(16))) uint8_t smem[ShLds<CB>::BYTES];
+    sddmm_shared_unit<CB, AT, GT>(p, p.unit_group[blockIdx.x], p.unit_chunk[blockIdx.x], smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,17 +1209,17 @@ __global__ __launch_bounds__(256) void quantize_cols_finalize_kernel(int N, floa
 __global__ void warm_up_kernel() {}
 
 // ---------------------------------------------------------------------------
-// Batched persistent kernel: every workgroup (one wave) takes work units from a
-// shared counter until the list is exhausted, so one launch covers the
+// Batched persistent kernel: every workgroup (kBatchWaves waves) takes work
+// units from a shared counter until the list is exhausted, so one launch covers the
 // forward SpMM, the transposed SpMM and the SDDMM of many layers.  Units are
 // sorted heaviest first on the host.  counters[0] hands out units,
 // counters[1] counts finished workgroups; the last one resets both, so the
 // next launch (stream-ordered) starts from zero without a memset.
-// Variants (all rows_per_wave 16, so that the kernel's register budget -- the
-// maximum over its cases -- stays at the level of one 16-row standalone
+// Variants (SpMM on 16-row plans only, so that the kernel's register budget --
+// the maximum over its cases -- stays at the level of one 16-row standalone
 // variant and several waves fit on each SIMD): 0..2 forward f16 x f16 with
 // tile_n 16/32/64, 3..4 transposed f16 x f32 with tile_n 16/32, 5 SDDMM with
-// an f32 G, 6 SDDMM with an f16 G.
+// an f32 G, 6 SDDMM with an f16 G (on the plan's 64-row SDDMM tiling).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1306,54 +1271,46 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     p.n_blocks_per_group_max = 0;
     p.unit_group = nullptr;
     p.unit_chunk = nullptr;
-    p.unit_panel = nullptr;
     p.g_bytes = d.a_bytes;
     p.x_bytes = d.b_bytes;
+    p.dst = d.dst;
     return p;
 }
 
-#define TCA_FWD_CASE(id, R, T)                                                                          \
-    case id:                                                                                            \
-        {                                                                                               \
-            f32x4 acc[R / 16][T / 16];                                                                  \
-            const SpmmParams sp = to_spmm(d);                                                           \
-            spmm_unit<R, T, TCA_F16, TCA_F16, false>(sp, un.g, un.t, un.c_lo, un.c_hi, lds_b, lds_a, acc); \
-            spmm_store_direct<R, T>(sp, un.g, un.t, acc, un.atomic != 0);                               \
-        }                                                                                               \
-        break;
-#define TCA_BWD_CASE(id, R, T)                                                                          \
-    case id:                                                                                            \
-        {                                                                                               \
-            f32x4 acc[R / 16][T / 16];                                                                  \
-            const SpmmParams sp = to_spmm(d);                                                           \
-            spmm_unit<R, T, TCA_F16, TCA_F32, true>(sp, un.g, un.t, un.c_lo, un.c_hi, lds_b, lds_a, acc); \
-            spmm_store_direct<R, T>(sp, un.g, un.t, acc, un.atomic != 0);                               \
-        }                                                                                               \
-        break;
+// The batch kernel's workgroup: kBatchWaves waves share every unit (SpMM:
+// chunk range split + LDS reduction; SDDMM: contraction split over N).
+constexpr int kBatchWaves = 4;
+constexpr int kBatchSlot = SpmmLds<16, 64, TCA_F16>::SLOT;   // largest SpMM variant's per-wave LDS
+constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES ? kBatchWaves * kBatchSlot
+                                                                                 : ShLds<kBatchSddmmCB>::BYTES;
 
-__global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restrict__ items,
-                                                   const BatchUnitDev *__restrict__ units, int n_units, int *counters)
+__global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItemDev *__restrict__ items,
+                                                                 const BatchUnitDev *__restrict__ units, int n_units,
+                                                                 int *counters)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[32 * tr_pitch<64>() * 2];
-    __shared__ __attribute__((aligned(16))) uint8_t lds_a[512 * 2 > 16 * 33 * 4 ? 512 * 2 : 16 * 33 * 4];
-    const int lane = threadIdx.x;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kBatchLds];
+    __shared__ int next_unit;
     for (;;) {
-        int u = 0;
-        if (lane == 0) u = atomicAdd(&counters[0], 1);
-        u = __builtin_amdgcn_readfirstlane(u);   // uniform: unit and item descriptors load into SGPRs
+        if (threadIdx.x == 0) next_unit = atomicAdd(&counters[0], 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
+        __syncthreads();                                            // all have read next_unit
         if (u >= n_units) break;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
-            TCA_FWD_CASE(0, 16, 16) TCA_FWD_CASE(1, 16, 32) TCA_FWD_CASE(2, 16, 64)
-            TCA_BWD_CASE(3, 16, 16) TCA_BWD_CASE(4, 16, 32)
-        case 5: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, 0, reinterpret_cast<float *>(lds_a)); break;
-        case 6: sddmm_unit<16, kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, 0, reinterpret_cast<float *>(lds_a)); break;
+        case 0: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 1: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 2: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 3: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 4: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
+        case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next unit
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         __threadfence();
         if (atomicAdd(&counters[1], 1) == (int)gridDim.x - 1) {
             atomicExch(&counters[0], 0);
@@ -1361,8 +1318,6 @@ __global__ __launch_bounds__(64) void batch_kernel(const BatchItemDev *__restric
         }
     }
 }
-#undef TCA_FWD_CASE
-#undef TCA_BWD_CASE
 
 // zero the outputs of split batch items (blockIdx.y = descriptor)
 __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
@@ -1377,8 +1332,8 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
 {
-    if (rpw != 16) return -1;
     if (op == TCA_OP_SPMM) {
+        if (rpw != 16) return -1;
         if (a_dtype != TCA_F16) return -1;
         if (!transposed && b_dtype == TCA_F16) return tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
         if (transposed && b_dtype == TCA_F32) return tile_n == 16 ? 3 : tile_n == 32 ? 4 : -1;
@@ -1394,7 +1349,7 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
 int batch_blocks_per_cu()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 64, 0) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 64 * kBatchWaves, 0) != hipSuccess || n < 1) n = 1;
     return n;
 }
 
@@ -1406,7 +1361,7 @@ tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, in
         TCA_HIP_TRY(hipGetLastError());
     }
     if (n_units <= 0) return TCA_OK;
-    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(64), 0, stream, items, units, n_units, counters);
+    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
@@ -1580,63 +1535,47 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
 
 namespace {
 using SddmmFn = void (*)(SddmmParams);
-constexpr int kSddmmCB = 4;
 
-template <tca_dtype AT, tca_dtype GT>
-SddmmFn pick_sddmm_rpw(int rpw)
+template <int CB>
+SddmmFn pick_sddmm(tca_dtype g_dtype, tca_dtype x_dtype)
 {
-    switch (rpw) {
-    case 16: return sddmm_kernel<16, kSddmmCB, AT, GT>;
-    case 32: return sddmm_kernel<32, kSddmmCB, AT, GT>;
-    case 64: return sddmm_kernel<64, kSddmmCB, AT, GT>;
-    }
+    if (x_dtype == TCA_F16 && g_dtype == TCA_F16) return sddmm_shared_kernel<CB, TCA_F16, TCA_F16>;
+    if (x_dtype == TCA_F16 && g_dtype == TCA_F32) return sddmm_shared_kernel<CB, TCA_F16, TCA_F32>;
+    if (x_dtype == TCA_BF16 && g_dtype == TCA_BF16) return sddmm_shared_kernel<CB, TCA_BF16, TCA_BF16>;
+    if (x_dtype == TCA_BF16 && g_dtype == TCA_F32) return sddmm_shared_kernel<CB, TCA_BF16, TCA_F32>;
     return nullptr;
 }
 }  // namespace
 
+// SDDMM on the plan's 64-row tiling (sddmm_tiling), dW written in the plan's
+// tiled order.  Blocks of 4 chunks per workgroup, or 2 when that leaves fewer
+// than kSddmmMinGroups workgroups.
+static constexpr int kSddmmMinGroups = 1024;
+
 tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype, const void *X,
                         int ldx, int N, float *out, hipStream_t stream)
 {
-    const PlanTables &t = p->host;
-    SddmmFn fn = nullptr;
-    const int kr = t.rpw <= 64 ? t.rpw : 32;
-    if (x_dtype == TCA_F16 && (g_dtype == TCA_F16)) fn = pick_sddmm_rpw<TCA_F16, TCA_F16>(kr);
-    else if (x_dtype == TCA_F16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_F16, TCA_F32>(kr);
-    else if (x_dtype == TCA_BF16 && g_dtype == TCA_BF16) fn = pick_sddmm_rpw<TCA_BF16, TCA_BF16>(kr);
-    else if (x_dtype == TCA_BF16 && g_dtype == TCA_F32) fn = pick_sddmm_rpw<TCA_BF16, TCA_F32>(kr);
-    if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: unsupported dtype pair");
+    if (!pick_sddmm<4>(g_dtype, x_dtype)) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: unsupported dtype pair");
     if (p->transposed) return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: needs a forward plan");
-    // work units: (group, first chunk, first panel) in heavy-first group order;
-    // plans wider than 64 rows run 32-row panel pairs of a group per unit
-    const int krpw = t.rpw <= 64 ? t.rpw : 32;
-    tca_plan *mp = const_cast<tca_plan *>(p);
-    if (!mp->d_sddmm_units) {
-        std::vector<int32_t> ug, uc, up;
-        for (int gi = 0; gi < t.n_groups; gi++) {
-            const int g = t.work_order[gi];
-            for (int c = t.grp_chunk_ptr[g]; c < t.grp_chunk_ptr[g + 1]; c += kSddmmCB)
-                for (int q = 0; q < t.rpw / krpw; q++) {
-                    if (g * t.rpw + q * krpw >= t.n_rows) break;
-                    ug.push_back(g);
-                    uc.push_back(c);
-                    up.push_back(q * krpw / 16);
-                }
-        }
-        mp->n_sddmm_units = (int)ug.size();
-        if (mp->n_sddmm_units > 0) {
-            TCA_HIP_TRY(hipMalloc(&mp->d_sddmm_units, sizeof(int32_t) * 3 * ug.size()));
-            int32_t *d = static_cast<int32_t *>(mp->d_sddmm_units);
-            TCA_HIP_TRY(hipMemcpy(d, ug.data(), sizeof(int32_t) * ug.size(), hipMemcpyHostToDevice));
-            TCA_HIP_TRY(hipMemcpy(d + ug.size(), uc.data(), sizeof(int32_t) * uc.size(), hipMemcpyHostToDevice));
-            TCA_HIP_TRY(hipMemcpy(d + 2 * ug.size(), up.data(), sizeof(int32_t) * up.size(), hipMemcpyHostToDevice));
-        }
+    const tca_plan *aux = nullptr;
+    const int32_t *dst = nullptr;
+    tca_status st = sddmm_tiling(p, &aux, &dst);
+    if (st != TCA_OK) return st;
+    const PlanTables &t = aux->host;
+    const int32_t *units = nullptr;
+    int n_units = 0, cb = 4;
+    st = sddmm_units(aux, 4, &units, &n_units);
+    if (st == TCA_OK && n_units < kSddmmMinGroups) {
+        cb = 2;
+        st = sddmm_units(aux, 2, &units, &n_units);
     }
-    if (mp->n_sddmm_units == 0) return TCA_OK;
+    if (st != TCA_OK) return st;
+    if (n_units == 0) return TCA_OK;
     SddmmParams prm;
-    prm.grp_chunk_ptr = p->d_grp_chunk_ptr;
-    prm.chunk_cols = p->d_chunk_cols;
-    prm.seg_ptr = p->d_seg_ptr;
-    prm.entry_slot = p->d_entry_slot;
+    prm.grp_chunk_ptr = aux->d_grp_chunk_ptr;
+    prm.chunk_cols = aux->d_chunk_cols;
+    prm.seg_ptr = aux->d_seg_ptr;
+    prm.entry_slot = aux->d_entry_slot;
     prm.G = G;
     prm.X = X;
     prm.out = out;
@@ -1646,18 +1585,15 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     prm.ldx = ldx;
     prm.rpw = t.rpw;
     prm.n_blocks_per_group_max = 0;
-    prm.unit_group = static_cast<const int32_t *>(mp->d_sddmm_units);
-    prm.unit_chunk = static_cast<const int32_t *>(mp->d_sddmm_units) + mp->n_sddmm_units;
-    prm.unit_panel = static_cast<const int32_t *>(mp->d_sddmm_units) + 2 * mp->n_sddmm_units;
+    prm.unit_group = units;
+    prm.unit_chunk = units + n_units;
+    prm.dst = dst;
     prm.g_bytes = operand_bytes(t.n_rows, N, ldg, g_dtype == TCA_F32 ? 4 : 2);
     prm.x_bytes = operand_bytes(t.n_cols, N, ldx, 2);
     if (prm.g_bytes == 0 || prm.x_bytes == 0)
         return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: G or X reaches 2 GiB (32-bit offsets)");
-    // split the contraction over N across waves while the grid is small
-    const int steps = N / 32;
-    int WS = 1;
-    while (WS < kSddmmMaxWaves && (long)mp->n_sddmm_units * WS < 4096 && steps / (2 * WS) >= 4) WS *= 2;
-    hipLaunchKernelGGL(fn, dim3(mp->n_sddmm_units), dim3(64 * WS), 0, stream, prm);
+    SddmmFn fn = cb == 4 ? pick_sddmm<4>(g_dtype, x_dtype) : pick_sddmm<2>(g_dtype, x_dtype);
+    hipLaunchKernelGGL(fn, dim3(n_units), dim3(256), 0, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
