@@ -60,8 +60,9 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
-                                                     " (batched SpMM: rows_per_wave 16; f16 A with f16 B forward, tile_n 16/32/64; "
-                                                     "f16 A with f32 B transposed, tile_n 16/32)");
+                                                     " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
+                                                     "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 B transposed "
+                                                     "on rows_per_wave 16 with tile_n 16/32/64)");
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;

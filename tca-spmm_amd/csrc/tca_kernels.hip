@@ -1215,11 +1215,11 @@ __global__ void warm_up_kernel() {}
 // sorted heaviest first on the host.  counters[0] hands out units,
 // counters[1] counts finished workgroups; the last one resets both, so the
 // next launch (stream-ordered) starts from zero without a memset.
-// Variants (SpMM on 16-row plans only, so that the kernel's register budget --
-// the maximum over its cases -- stays at the level of one 16-row standalone
-// variant and several waves fit on each SIMD): 0..2 forward f16 x f16 with
-// tile_n 16/32/64, 3..4 transposed f16 x f32 with tile_n 16/32, 5 SDDMM with
-// an f32 G, 6 SDDMM with an f16 G (on the plan's 64-row SDDMM tiling).
+// Variants (few and small, so that the kernel's register budget -- the maximum
+// over its cases -- keeps three waves on each SIMD): forward f16 x f16 on
+// 16-row plans with tile_n 16/32/64 (0..2) and on 32-row plans with tile_n 32
+// (8); transposed f16 x f32 on 16-row plans with tile_n 16/32/64 (3, 4, 7);
+// SDDMM with an f32 (5) or f16 (6) G on the plan's 64-row SDDMM tiling.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1280,7 +1280,9 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
 // The batch kernel's workgroup: kBatchWaves waves share every unit (SpMM:
 // chunk range split + LDS reduction; SDDMM: contraction split over N).
 constexpr int kBatchWaves = 4;
-constexpr int kBatchSlot = SpmmLds<16, 64, TCA_F16>::SLOT;   // largest SpMM variant's per-wave LDS
+constexpr int kBatchSlot = SpmmLds<16, 64, TCA_F16>::SLOT > SpmmLds<32, 32, TCA_F16>::SLOT   // largest SpMM
+                               ? SpmmLds<16, 64, TCA_F16>::SLOT                           // variant's per-wave LDS
+                               : SpmmLds<32, 32, TCA_F16>::SLOT;
 constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES ? kBatchWaves * kBatchSlot
                                                                                  : ShLds<kBatchSddmmCB>::BYTES;
 
@@ -1304,6 +1306,8 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         case 2: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 3: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 4: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 7: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 8: spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
         case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
@@ -1333,10 +1337,12 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
 {
     if (op == TCA_OP_SPMM) {
-        if (rpw != 16) return -1;
         if (a_dtype != TCA_F16) return -1;
-        if (!transposed && b_dtype == TCA_F16) return tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
-        if (transposed && b_dtype == TCA_F32) return tile_n == 16 ? 3 : tile_n == 32 ? 4 : -1;
+        if (!transposed && b_dtype == TCA_F16) {
+            if (rpw == 16) return tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
+            if (rpw == 32) return tile_n == 32 ? 8 : -1;
+        }
+        if (transposed && b_dtype == TCA_F32 && rpw == 16) return tile_n == 16 ? 3 : tile_n == 32 ? 4 : tile_n == 64 ? 7 : -1;
         return -1;
     }
     if (op == TCA_OP_SDDMM && b_dtype == TCA_F16 && !transposed) {

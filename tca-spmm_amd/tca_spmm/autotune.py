@@ -42,8 +42,8 @@ def heuristic(M, K, nnz, N):
     return rpw, tile
 
 
-BATCH_RPW = (16,)             # variants compiled into the persistent batch kernel
-BATCH_TILES = {"spmm": (16, 32, 64), "spmm_t": (16, 32)}
+# (rows_per_wave -> tile_n values) compiled into the persistent batch kernel
+BATCH_VARIANTS = {"spmm": {16: (16, 32, 64), 32: (32,)}, "spmm_t": {16: (16, 32, 64)}}
 
 
 def choice_for(M, K, nnz, N, op="spmm", batch=False):
@@ -61,11 +61,11 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False):
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
         for c in best.get("candidates", [best]):
-            if not batch or (c["rows_per_wave"] in BATCH_RPW and c["tile_n"] in BATCH_TILES[op]):
+            if not batch or c["tile_n"] in BATCH_VARIANTS[op].get(c["rows_per_wave"], ()):
                 return c["rows_per_wave"], c["tile_n"]
     rpw, tile = heuristic(M, K, nnz, N)
-    if batch:
-        rpw, tile = BATCH_RPW[0], min(tile, max(BATCH_TILES[op]))
+    if batch and tile not in BATCH_VARIANTS[op].get(rpw, ()):
+        rpw, tile = 16, min(tile, 64)
     return rpw, tile
 
 
