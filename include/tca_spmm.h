@@ -131,6 +131,16 @@ tca_status tca_plan_create_host(int n_rows, int n_cols, int nnz, const int32_t *
  * both directions. */
 tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_per_wave, tca_plan **plan_t);
 tca_status tca_plan_destroy(tca_plan *plan);
+/* Transpose of a CSR pattern held on the device (its CSC), built on the device:
+ * d_col_offset[n_cols+1], d_row_idx[nnz] (rows ascending within each column) and,
+ * unless NULL, d_perm[nnz] = the CSR index of each CSC entry (the value
+ * permutation).  Replaces the reference's host transpose of dense matrices
+ * (src/miscalleneous.cu:146-157) for patterns that live on the GPU; allocates
+ * stream-ordered scratch. */
+tca_status tca_csr_transpose_device(int n_rows, int n_cols, int nnz, const int32_t *d_row_offset,
+                                    const int32_t *d_col_idx, int32_t *d_col_offset, int32_t *d_row_idx,
+                                    int32_t *d_perm, void *stream);
+
 tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *info);
 /* Host copies of the tables (for the host-rebuild check).  Sizes:
  * grp_chunk_ptr[n_groups+1], chunk_cols[32*n_chunks], seg_ptr[n_chunks*rpw/16+1],

@@ -61,12 +61,12 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
                                                      " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
-                                                     "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 B transposed "
-                                                     "on rows_per_wave 16 with tile_n 16/32/64)");
+                                                     "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 or f16 B "
+                                                     "transposed on rows_per_wave 16 with tile_n 16/32/64)");
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;
-            d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, it.b_dtype == TCA_F32 ? 4 : 2);
+            d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, it.b_dtype == TCA_F32 ? 4 : 2);   // B: f32 or f16
             if (d.b_bytes == 0 || (long)t.nnz * 4 >= (1L << 31))
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " reaches 2 GiB");
             for (int gi = 0; gi < t.n_groups; gi++) {

@@ -163,7 +163,8 @@ extern "C" tca_status tca_plan_destroy(tca_plan *plan)
     int cur = 0;
     bool switched = false;
     if ((plan->d_base || plan->d_sddmm_dst || plan->d_sddmm_units[0] || plan->d_sddmm_units[1] ||
-         !plan->unit_tables.empty()) && hipGetDevice(&cur) == hipSuccess && cur != plan->device)
+         plan->d_sddmm_units[2] || !plan->unit_tables.empty()) &&
+        hipGetDevice(&cur) == hipSuccess && cur != plan->device)
         switched = hipSetDevice(plan->device) == hipSuccess;
     if (plan->d_base) (void)hipFree(plan->d_base);
     if (plan->d_sddmm_dst) (void)hipFree(plan->d_sddmm_dst);
@@ -221,7 +222,7 @@ tca_status sddmm_tiling(const tca_plan *cp, const tca_plan **aux, const int32_t 
 tca_status sddmm_units(const tca_plan *cp, int cb, const int32_t **units, int *n)
 {
     tca_plan *p = const_cast<tca_plan *>(cp);
-    const int slot = cb == 4 ? 0 : 1;
+    const int slot = cb == 4 ? 0 : cb == 2 ? 1 : 2;
     if (!p->d_sddmm_units[slot]) {
         const PlanTables &t = p->host;
         std::vector<int32_t> ug, uc;

@@ -63,9 +63,9 @@ struct tca_plan {
     tca_plan *sddmm_aux = nullptr;
     int32_t *d_sddmm_dst = nullptr;
     // SDDMM work units (group, first chunk) of this plan as an SDDMM tiling,
-    // for 4- and 2-chunk blocks, built on first use
-    void *d_sddmm_units[2] = {nullptr, nullptr};
-    int n_sddmm_units[2] = {0, 0};
+    // for 4-, 2- and 1-chunk blocks, built on first use
+    void *d_sddmm_units[3] = {nullptr, nullptr, nullptr};
+    int n_sddmm_units[3] = {0, 0, 0};
     // SpMM unit tables (group, pieces, c_lo, c_hi), one per split factor
     struct UnitTable {
         int pieces = 1, count = 0;
@@ -97,8 +97,8 @@ tca_status launch_warm_up(hipStream_t stream);
 // *dst maps its entries to p's tiled order (null when *aux == p).
 constexpr int kSddmmRows = 64;
 tca_status sddmm_tiling(const tca_plan *p, const tca_plan **aux, const int32_t **dst);
-// SDDMM work units (group, first chunk) of a 64-row tiling for blocks of cb (4
-// or 2) chunks, heavy groups first: n pairs at *units (groups, then chunks).
+// SDDMM work units (group, first chunk) of a 64-row tiling for blocks of cb (4,
+// 2 or 1) chunks, heavy groups first: n pairs at *units (groups, then chunks).
 tca_status sddmm_units(const tca_plan *aux, int cb, const int32_t **units, int *n);
 // Bytes from the first to the last element of a rows x N row-major operand with
 // leading dimension ld; 0 when that is not below 2^31 (the kernels address

@@ -988,7 +988,8 @@ struct ShLds {
     static constexpr int XT = CB * 32 * kShPitch * 2;           // one X tile (bytes)
     static constexpr int STAGE = 16 * 33 * 4;                   // one wave's result tile
     static constexpr int BYTES = 2 * XT + 4 * STAGE;
-    static constexpr int XP = CB * 32 * 4 / 256;                // X pieces per thread and step
+    static constexpr int PIECES = CB * 32 * 4;                  // 16-byte X pieces per step
+    static constexpr int XP = (PIECES + 255) / 256;             // X pieces per thread and step
 };
 
 template <int CB, tca_dtype AT, tca_dtype GT>
@@ -1007,10 +1008,12 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     // this thread's X pieces: tile row (chunk b = row/32), 8-n octet q
     uint32_t xsrc[L::XP];
     int xdst[L::XP];
+    bool xval[L::XP];
 #pragma unroll
     for (int j = 0; j < L::XP; j++) {
         const int pc = tid + 256 * j, row = pc >> 2, q = pc & 3, b = row >> 5;
-        const int col = b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + (row & 31)] : -1;
+        xval[j] = pc < L::PIECES;
+        const int col = xval[j] && b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + (row & 31)] : -1;
         xsrc[j] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + 8u * q) * 2u : kOOB;
         xdst[j] = (row * kShPitch + 8 * q) * 2;
     }
@@ -1035,7 +1038,8 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     auto put_x = [&](int s) {
         uint8_t *buf = smem + (s & 1) * L::XT;
 #pragma unroll
-        for (int j = 0; j < L::XP; j++) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[j];
+        for (int j = 0; j < L::XP; j++)
+            if (xval[j]) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[j];
     };
     // prologue: X of step 0 in LDS, X of step 1 and G of steps 0, 1 in registers
     load_x(0);
@@ -1218,8 +1222,9 @@ __global__ void warm_up_kernel() {}
 // Variants (few and small, so that the kernel's register budget -- the maximum
 // over its cases -- keeps three waves on each SIMD): forward f16 x f16 on
 // 16-row plans with tile_n 16/32/64 (0..2) and on 32-row plans with tile_n 32
-// (8); transposed f16 x f32 on 16-row plans with tile_n 16/32/64 (3, 4, 7);
-// SDDMM with an f32 (5) or f16 (6) G on the plan's 64-row SDDMM tiling.
+// (8); transposed f16 x f32 on 16-row plans with tile_n 16/32/64 (3, 4, 7) and
+// f16 x f16 (9..11); SDDMM with an f32 (5) or f16 (6) G on the plan's 64-row
+// SDDMM tiling.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1308,6 +1313,9 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         case 4: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 7: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 8: spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 9: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 10: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 11: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
         case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
@@ -1343,6 +1351,7 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
             if (rpw == 32) return tile_n == 32 ? 8 : -1;
         }
         if (transposed && b_dtype == TCA_F32 && rpw == 16) return tile_n == 16 ? 3 : tile_n == 32 ? 4 : tile_n == 64 ? 7 : -1;
+        if (transposed && b_dtype == TCA_F16 && rpw == 16) return tile_n == 16 ? 9 : tile_n == 32 ? 10 : tile_n == 64 ? 11 : -1;
         return -1;
     }
     if (op == TCA_OP_SDDMM && b_dtype == TCA_F16 && !transposed) {
@@ -1554,8 +1563,8 @@ SddmmFn pick_sddmm(tca_dtype g_dtype, tca_dtype x_dtype)
 }  // namespace
 
 // SDDMM on the plan's 64-row tiling (sddmm_tiling), dW written in the plan's
-// tiled order.  Blocks of 4 chunks per workgroup, or 2 when that leaves fewer
-// than kSddmmMinGroups workgroups.
+// tiled order.  Blocks of 4 chunks per workgroup, halved (to 2, then 1) while
+// that leaves fewer than kSddmmMinGroups workgroups.
 static constexpr int kSddmmMinGroups = 1024;
 
 tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype, const void *X,
@@ -1571,9 +1580,9 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     const int32_t *units = nullptr;
     int n_units = 0, cb = 4;
     st = sddmm_units(aux, 4, &units, &n_units);
-    if (st == TCA_OK && n_units < kSddmmMinGroups) {
-        cb = 2;
-        st = sddmm_units(aux, 2, &units, &n_units);
+    while (st == TCA_OK && n_units < kSddmmMinGroups && cb > 1) {
+        cb >>= 1;
+        st = sddmm_units(aux, cb, &units, &n_units);
     }
     if (st != TCA_OK) return st;
     if (n_units == 0) return TCA_OK;
@@ -1598,7 +1607,8 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     prm.x_bytes = operand_bytes(t.n_cols, N, ldx, 2);
     if (prm.g_bytes == 0 || prm.x_bytes == 0)
         return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: G or X reaches 2 GiB (32-bit offsets)");
-    SddmmFn fn = cb == 4 ? pick_sddmm<4>(g_dtype, x_dtype) : pick_sddmm<2>(g_dtype, x_dtype);
+    SddmmFn fn = cb == 4 ? pick_sddmm<4>(g_dtype, x_dtype) : cb == 2 ? pick_sddmm<2>(g_dtype, x_dtype)
+                                                                   : pick_sddmm<1>(g_dtype, x_dtype);
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(256), 0, stream, prm);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;

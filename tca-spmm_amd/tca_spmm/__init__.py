@@ -10,12 +10,12 @@ from . import autotune, gen, workloads  # noqa: F401
 lib()  # fail loudly at import when the native library is absent
 
 try:  # torch-facing API (torch is the device-memory layer, not the compute)
-    from .core import (Plan, hipblaslt_gemm, quantize_cols_e4m3, quantize_rows_e4m3,  # noqa: F401
-                       rocsparse_spmm, sddmm, spmm)
+    from .core import (Plan, PlanCache, csr_transpose_device, hipblaslt_gemm,  # noqa: F401
+                       quantize_cols_e4m3, quantize_rows_e4m3, rocsparse_spmm, sddmm, spmm)
     from .autograd import SparseLinear, SpmmFunction, sparse_matmul  # noqa: F401
     from .batch import Batch, sddmm_item, spmm_item  # noqa: F401
 except ImportError:  # pragma: no cover - torch absent
     pass
 
-__all__ = ["Plan", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
+__all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
            "hipblaslt_gemm", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "TcaError", "F16", "BF16", "E4M3", "F32"]

@@ -1,9 +1,11 @@
 """Sparse-weight training: Y = W X with a CSR weight W, differentiable in W and X.
 
 Backward (hand-written, both on the C ABI):
-  dX = W^T G        transposed-plan SpMM (G read as float32, rounded to the
-                    weight's MFMA type on load)
+  dX = W^T G        transposed-plan SpMM
   dW = SDDMM(G, X^T) sampled on W's pattern, FP32, in tiled order
+Both read G in the weight's MFMA type: an FP32 G is rounded to it once (round to
+nearest even, the same bits the kernels would produce rounding on load), which
+halves the bytes both kernels gather.
 W's values live in *tiled order* (the plan's entry order) as the parameter,
 so no permutation runs per step; `SparseLinear.csr_values()` exports them.
 """
@@ -25,8 +27,8 @@ class SpmmFunction(torch.autograd.Function):
     def backward(ctx, G):
         w_tiled, X = ctx.saved_tensors
         G = G.contiguous()
-        if G.dtype != torch.float32:
-            G = G.float()
+        if G.dtype != w_tiled.dtype:
+            G = G.to(w_tiled.dtype)
         dW = dX = None
         if ctx.needs_input_grad[0]:
             dW = sddmm(ctx.plan, G, X).to(w_tiled.dtype)

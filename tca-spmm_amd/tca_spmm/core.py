@@ -45,8 +45,9 @@ class Plan:
 
     rows_per_wave in {16, 32, 64}: consecutive rows sharing one condensed
     column list (16-row MFMA panels, 1/2/4 per wave); 128 or 256: one
-    workgroup of rows/32 waves (2 panels each) shares every gathered B chunk.  device=False builds a
-    host-only plan (inspection/export only; launches reject it).
+    workgroup of rows/32 waves (2 panels each) shares every gathered B chunk.
+    device=False builds a host-only plan (inspection/export only; launches
+    reject it).
     """
 
     def __init__(self, n_rows, n_cols, row_offset, col_idx, rows_per_wave=16, device=True, _handle=None):
@@ -130,6 +131,62 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class PlanCache:
+    """Plans of fixed sparsity patterns, built once and reused: a training loop
+    over pattern-static sparse layers keeps the host preprocessing (tiling,
+    transposed tiling) out of its steps.  Keyed by the pattern's shape,
+    rows_per_wave and a BLAKE2b digest of row_offset and col_idx."""
+
+    def __init__(self):
+        self._plans = {}
+
+    @staticmethod
+    def key(n_rows, n_cols, row_offset, col_idx, rows_per_wave):
+        import hashlib
+        h = hashlib.blake2b(digest_size=16)
+        h.update(np.ascontiguousarray(row_offset, dtype=np.int32).tobytes())
+        h.update(np.ascontiguousarray(col_idx, dtype=np.int32).tobytes())
+        return (int(n_rows), int(n_cols), int(rows_per_wave), h.hexdigest())
+
+    def get(self, n_rows, n_cols, row_offset, col_idx, rows_per_wave=16):
+        k = self.key(n_rows, n_cols, row_offset, col_idx, rows_per_wave)
+        p = self._plans.get(k)
+        if p is None:
+            p = self._plans[k] = Plan(n_rows, n_cols, row_offset, col_idx, rows_per_wave=rows_per_wave)
+        return p
+
+    def get_transpose(self, n_rows, n_cols, row_offset, col_idx, rows_per_wave=16, rows_per_wave_t=16):
+        """(forward plan, transposed plan reading the forward plan's tiled values)."""
+        fwd = self.get(n_rows, n_cols, row_offset, col_idx, rows_per_wave)
+        k = self.key(n_rows, n_cols, row_offset, col_idx, rows_per_wave) + ("T", int(rows_per_wave_t))
+        t = self._plans.get(k)
+        if t is None:
+            t = self._plans[k] = fwd.transpose(rows_per_wave_t)
+        return fwd, t
+
+    def __len__(self):
+        return len(self._plans)
+
+    def clear(self):
+        self._plans.clear()
+
+
+def csr_transpose_device(row_offset, col_idx, n_cols, stream=None):
+    """CSC of a CSR pattern held in device int32 tensors, built on the device:
+    (col_offset[n_cols+1], row_idx[nnz], perm[nnz]) where perm[d] is the CSR
+    index of CSC entry d (rows ascending within each column)."""
+    n_rows = row_offset.numel() - 1
+    nnz = col_idx.numel()
+    dev = row_offset.device
+    col_offset = torch.empty(n_cols + 1, dtype=torch.int32, device=dev)
+    row_idx = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    perm = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    call("tca_csr_transpose_device", n_rows, n_cols, nnz, _ptr(row_offset), _ptr(col_idx) if nnz else ctypes.c_void_p(0),
+         _ptr(col_offset), _ptr(row_idx) if nnz else ctypes.c_void_p(0), _ptr(perm) if nnz else ctypes.c_void_p(0),
+         _stream(stream))
+    return col_offset, row_idx, perm
 
 
 def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, stream=None):

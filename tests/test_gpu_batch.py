@@ -101,3 +101,23 @@ def test_batch_rejects_unsupported(cuda):
     with pytest.raises(T.TcaError) as e:
         T.Batch([T.spmm_item(plan, w, B, C)])
     assert e.value.name == "TCA_ERR_UNSUPPORTED"
+
+
+def test_batch_with_fp16_gradient_matches_fp32_gradient(cuda):
+    # G rounded once to FP16 gives the same MFMA inputs as an FP32 G rounded on load
+    layers = _layers(cuda)
+    batch = T.Batch(_items(layers))
+    batch.run()
+    torch.cuda.synchronize()
+    ref = [(L["dX"].clone(), L["dW"].clone()) for L in layers]
+    items = []
+    for L in layers:
+        L["G16"] = L["G"].half()
+        L["dX"].zero_()
+        L["dW"].zero_()
+        items.append(T.spmm_item(L["plan_t"], L["w"], L["G16"], L["dX"], L["tile_t"]))
+        items.append(T.sddmm_item(L["plan"], L["G16"], L["X"], L["dW"]))
+    T.Batch(items).run()
+    torch.cuda.synchronize()
+    for L, (dx, dw) in zip(layers, ref):
+        assert torch.equal(L["dX"], dx) and torch.equal(L["dW"], dw)
