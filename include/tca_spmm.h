@@ -173,7 +173,9 @@ tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_til
 
 /* SDDMM on the plan's pattern: out_tiled[e] = sum_n G[row(e), n] * X[col(e), n]
  * (the dW of Y = W X with dY = G).  G: M x N row-major (ldg), X: K x N (ldx),
- * dtypes F16/BF16, or F32 (G) converted to F16 on load.  N % 32 == 0. */
+ * dtypes F16/BF16, or F32 (G) rounded to X's type on load.  N % 32 == 0.
+ * Runs on a 64-row tiling of the same pattern, built on the first call and
+ * kept with the plan; out_tiled is in the plan's own tiled order. */
 tca_status tca_sddmm(const tca_plan *plan, tca_dtype g_dtype, const void *d_G, int ldg,
                      tca_dtype x_dtype, const void *d_X, int ldx, int N, float *d_out_tiled, void *stream);
 
@@ -189,6 +191,10 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 
 /* ------------------------------------------------------------------------ */
 /* Batched persistent launch over many layers (one kernel, capturable).      */
+/* SPMM items: F16 values; forward with an F16 B on 16-row plans (tile_n     */
+/* 16/32/64) or 32-row plans (tile_n 32); transposed with an F32 or F16 B on  */
+/* 16-row plans (tile_n 16/32/64).  SDDMM items: F16 X, F32 or F16 G.  Other  */
+/* combinations return TCA_ERR_UNSUPPORTED.                                  */
 /* ------------------------------------------------------------------------ */
 typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
 typedef struct tca_batch_item {
