@@ -350,6 +350,21 @@ def main():
         us = s.elapsed_time(t) * 1e3 / 50
         per.append({"name": e["case"].A.name, "us": us, "tflops": e["flops"] / (us * 1e-6) / 1e12,
                     "hbm_frac": e["bytes"] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS})
+    # cold: a 512 MiB write between launches evicts L2 and the Infinity Cache, so
+    # every operand comes from HBM (event pair around the launch alone)
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
+    for e, row in zip(entries, per):
+        cold = []
+        for _ in range(10):
+            flush.fill_(1)
+            s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+            t.record(stream)
+            torch.cuda.synchronize()
+            cold.append(s.elapsed_time(t) * 1e3)
+        row["cold_us"] = float(np.median(cold))
+    del flush
     extra["per_matrix"] = per
 
     # library baselines on the same inputs
@@ -422,6 +437,15 @@ def main():
                 O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B, cores)
             reps += 1
         dt = time.perf_counter() - t0
+        # one thread on the first matrix, for the scaling of the CPU figure
+        A0 = entries[0]["case"].A
+        reps1, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 2.0:
+            O.spmm_f32(A0.n_rows, N, A0.row_offset, A0.col_idx, A0.values, entries[0]["case"].B, 1)
+            reps1 += 1
+        dt1 = time.perf_counter() - t1
+        extra["cpu_one_thread"] = {"matrix": A0.name, "ms": dt1 / reps1 * 1e3,
+                                   "tflops": entries[0]["flops"] * reps1 / dt1 / 1e12}
         cpu = {"value": total_flops * reps / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
                "sample": f"the same ten-matrix suite (N=256) repeated {reps}x in {dt:.1f}s "
                          f"({dt / reps * 1e3:.2f} ms per suite), FP32 accumulate, OpenMP over rows"}

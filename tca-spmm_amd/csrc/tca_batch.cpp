@@ -74,7 +74,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 const int nch = t.grp_chunk_ptr[g + 1] - t.grp_chunk_ptr[g];
                 for (int tt = 0; tt < d.n_tiles; tt++) {
                     units.push_back({i, g, tt, t.grp_chunk_ptr[g], t.grp_chunk_ptr[g + 1], 0, 0, 0});
-                    cost.push_back(nch * (2.0 + ppg * tile / 16.0) + 4.0);   // ~ MFMAs + staging per chunk
+                    // ~ staging + MFMAs per chunk; an FP32 B gathers twice the bytes
+                    cost.push_back(nch * (2.0 + ppg * tile / 16.0 * (it.b_dtype == TCA_F32 ? 1.5 : 1.0)) + 4.0);
                 }
             }
         } else if (it.op == TCA_OP_SDDMM) {
