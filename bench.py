@@ -4,13 +4,15 @@
 Workload (one "step"): the FFN-shaped GPU-forward suite -- CSR A of 2048x512
 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 (lognormal(0.5) row lengths,
 values N(0,0.02)), B = K x 256 N(0,1), FP16 inputs, FP32 accumulate and output:
-ten SpMMs per step, inputs resident in HBM, the ten launches captured once in a
-HIP graph and replayed.  value = sum(2*nnz*N) / step time (TFLOP/s).
+ten SpMMs per step, inputs resident in HBM, run as ONE persistent batched
+launch (the ten products' work units drain through one grid), K launches back
+to back.  value = sum(2*nnz*N) / step time (TFLOP/s).
 
-Also reported: the HBM roofline of the SpMM kernel (algorithmic bytes per
-launch / measured launch time vs 8 TB/s), rocSPARSE and hipBLASLt on the same
-inputs, and the CPU oracle (FP32-accumulate C port of naive_gemm restricted to
-nonzeros) timed on the host cores.
+Also reported: the HBM roofline of that launch (the ten products' algorithmic
+bytes / its measured duration vs 8 TB/s), the ten tuned per-call launches
+(graph and eager), per-matrix warm and cold launch times, rocSPARSE and
+hipBLASLt on the same inputs, and the CPU oracle (FP32-accumulate C port of
+naive_gemm restricted to nonzeros) timed on the host cores.
 Prints one JSON line on stdout.
 """
 import argparse
@@ -68,14 +70,17 @@ def time_graph(torch, fn, iters):
     return s.elapsed_time(e) / iters
 
 
-def time_eager(torch, fn, iters):
-    """Mean ms of fn() launched eagerly (host launch cost included when the
-    GPU outruns the host)."""
+def time_eager(torch, fn, iters, backlog=False):
+    """Mean ms of fn() launched eagerly: host launch cost included when the
+    GPU outruns the host, unless `backlog` queues a spin kernel first so the
+    launches are all enqueued before the GPU reaches them."""
     for _ in range(3):
         fn()
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if backlog and hasattr(torch.cuda, "_sleep"):
+        torch.cuda._sleep(20_000_000)
     s.record(st)
     for _ in range(iters):
         fn()
@@ -140,9 +145,13 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     bt = T.Batch(items)
     plain_eager = time_eager(torch, per_call, 20)
     plain_graph = time_graph(torch, per_call, 20)
-    fused = time_graph(torch, bt.run, 50)
+    # the fused step is one batch launch (plus the fill kernel zeroing the
+    # outputs of split units): launched back to back, and replayed from a graph
+    fused = time_eager(torch, bt.run, 50, backlog=True)
+    fused_graph = time_graph(torch, bt.run, 50)
     per_op = {op: time_graph(torch, only(op), 20) for op in ("fwd", "dx", "dw")}
     return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph, "fused_ms": fused,
+            "fused_graph_ms": fused_graph,
             "speedup_vs_eager": plain_eager / fused, "speedup_vs_graph": plain_graph / fused,
             "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "per_call_graph_ms_by_op": per_op,
@@ -207,8 +216,8 @@ def bench_precision(torch, np, T, O, dev):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="batched,fwdbwd,precision",
-                    help="comma list of extra sections: batched, fwdbwd, precision")
+    ap.add_argument("--sections", default="fwdbwd,precision",
+                    help="comma list of extra sections: fwdbwd, precision")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -263,74 +272,93 @@ def main():
     total_flops = sum(e["flops"] for e in entries)
     total_bytes = sum(e["bytes"] for e in entries)
 
-    def step():
+    def per_call():
         for e in entries:
             T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
 
-    # parity gate before timing: every product against the CPU oracle
-    from oracle import oracle as O
-    step()
-    torch.cuda.synchronize()
-    worst = 0.0
+    # The step: the ten products as ONE persistent batched launch (their work
+    # units drain through one grid), the fastest way found to run them; the
+    # batch uses 16-row plans with 64-column tiles (autotune.choice_for(batch=True)).
+    bitems = []
     for e in entries:
         A = e["case"].A
-        ref = O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B)
-        got = e["C"].cpu().numpy()
-        rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        worst = max(worst, rel)
-        assert rel <= 1e-5, f"{A.name}: rel-L2 {rel} vs oracle"
+        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
+        if rpw_b == e["rpw"]:
+            bitems.append(T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], tile_b))
+        else:
+            pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
+            e["batch_keep"] = (pb, pb.pack(f16(A.values)))
+            bitems.append(T.spmm_item(pb, e["batch_keep"][1], e["B"], e["C"], tile_b))
+    batch = T.Batch(bitems)
+
+    def step():
+        batch.run()
+
+    # parity gate before timing: every product of the step, and of the per-call
+    # launches, against the CPU oracle
+    from oracle import oracle as O
+    refs = []
+    for e in entries:
+        A = e["case"].A
+        refs.append(O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B))
+    worst = 0.0
+    for run in (step, per_call):
+        for e in entries:
+            e["C"].fill_(float("nan"))
+        run()
+        torch.cuda.synchronize()
+        for e, ref in zip(entries, refs):
+            rel = float(np.linalg.norm(e["C"].cpu().numpy() - ref) / np.linalg.norm(ref))
+            worst = max(worst, rel)
+            assert rel <= 1e-5, f"{e['case'].A.name}: rel-L2 {rel} vs oracle"
     log(f"parity ok: worst rel-L2 vs CPU oracle {worst:.2e}")
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        step()
-    for _ in range(max(3, args.warmup // 4)):
-        graph.replay()
-    torch.cuda.synchronize()
 
-    # timed region: K graph replays, an event pair around each (same stream)
+    # timed region: K steps back to back on one stream, an event pair around
+    # them, behind a queue backlog so the host's enqueue cost stays off the GPU
+    # clock.  A step is one kernel launch, so no graph is needed to hide launch
+    # cost; the same step replayed from a HIP graph is reported beside it.
     K = args.steps
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     t_all0, t_all1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     if hasattr(torch.cuda, "_sleep"):
-        torch.cuda._sleep(20_000_000)        # queue backlog so host enqueue cost stays off the GPU clock
+        torch.cuda._sleep(20_000_000)
     wall0 = time.perf_counter()
     t_all0.record(stream)
-    for i in range(K):
-        starts[i].record(stream)
-        graph.replay()
-        ends[i].record(stream)
+    for _ in range(K):
+        step()
     t_all1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    ms_total = t_all0.elapsed_time(t_all1)
-    replay_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    ms_per_step = float(np.mean(replay_ms))
-    ms_per_step_total = ms_total / K
-    launch_us = ms_per_step * 1e3 / len(entries)
+    ms_per_step = t_all0.elapsed_time(t_all1) / K
     value = total_flops / (ms_per_step * 1e-3) / 1e12
+    # roofline of the step's kernel (batch_kernel, one launch per step): the
+    # ten products' algorithmic bytes / the launch's duration
+    launch_us = ms_per_step * 1e3
     achieved = total_bytes / (ms_per_step * 1e-3) / 1e9
+    graph_ms = time_graph(torch, step, K)
 
-    # eager (no graph) for reference
+    # the ten tuned per-call launches, captured in one graph (one stream) and eager
+    per_call_graph_ms = time_graph(torch, per_call, K)
     for _ in range(3):
-        step()
+        per_call()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(K):
-        step()
+        per_call()
     e1.record(stream)
     torch.cuda.synchronize()
     eager_ms = e0.elapsed_time(e1) / K
 
-    extra = {"graph_ms_per_step_incl_gaps": ms_per_step_total, "eager_ms_per_step": eager_ms,
-             "wall_s_timed": wall, "per_launch_us_mean": launch_us,
+    extra = {"graph_replay_ms_per_step": graph_ms, "wall_s_timed": wall,
+             "batch_units": batch.n_units, "batch_workgroups": batch.n_workgroups,
+             "per_call_graph_ms_per_step": per_call_graph_ms, "per_call_eager_ms_per_step": eager_ms,
+             "speedup_vs_per_call_graph": per_call_graph_ms / ms_per_step,
              "configs": [{"name": e["case"].A.name, "nnz": e["case"].A.nnz, "rows_per_wave": e["rpw"],
                           "tile_n": e["tile_n"]} for e in entries]}
 
@@ -394,31 +422,6 @@ def main():
         extra["speedup_vs_rocsparse"] = lib_ms["rocsparse"] / ms_per_step if lib_ms["rocsparse"] else None
         extra["speedup_vs_hipblaslt"] = lib_ms["hipblaslt"] / ms_per_step if lib_ms["hipblaslt"] else None
 
-    # the same ten products as ONE persistent launch
-    if "batched" in args.sections:
-        bitems = []
-        for e in entries:
-            A = e["case"].A
-            rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
-            if rpw_b == e["rpw"]:
-                bitems.append(T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], tile_b))
-            else:
-                pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
-                e["batch_keep"] = (pb, pb.pack(f16(A.values)))
-                bitems.append(T.spmm_item(pb, e["batch_keep"][1], e["B"], e["C"], tile_b))
-        bt = T.Batch(bitems)
-        us = time_graph(torch, lambda: bt.run(), 100) * 1e3
-        extra["batched_suite"] = {"us_per_launch": us, "tflops": total_flops / (us * 1e-6) / 1e12,
-                                  "hbm_frac": total_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                                  "units": bt.n_units, "workgroups": bt.n_workgroups}
-        bt.run()
-        torch.cuda.synchronize()
-        for e in entries:
-            A = e["case"].A
-            ref = O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B)
-            rel = float(np.linalg.norm(e["C"].cpu().numpy() - ref) / np.linalg.norm(ref))
-            assert rel <= 1e-5, f"batched {A.name}: rel-L2 {rel}"
-
     if "fwdbwd" in args.sections:
         extra["fwd_bwd_12_layers"] = bench_fwd_bwd(torch, np, T, O, dev)
     if "precision" in args.sections:
@@ -464,14 +467,14 @@ def main():
         "dtype": "f16",
         "data": "synthetic",
         "config": {"workload": "plain_gpu: CSR 2048x512 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 "
-                               "(lognormal(0.5) rows), B Kx256, FP16 in, FP32 accumulate/out; 10 SpMMs per step, "
-                               "one HIP graph replay",
+                               "(lognormal(0.5) rows), B Kx256, FP16 in, FP32 accumulate/out; 10 SpMMs per step "
+                               "as one persistent batched launch",
                    "N": N, "matrices": len(entries), "nnz_total": int(sum(e["case"].A.nnz for e in entries)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("spmm_plain_gpu_suite_traffic"),
-                     "kernel": "spmm_kernel (10 launches per step; algorithmic bytes summed over the step / "
-                               "graph replay time)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch_plain_gpu_suite_traffic"),
+                     "kernel": "batch_kernel (the step's one launch: ten products' algorithmic bytes / the "
+                               "launch's duration, event pair around K back-to-back launches)",
                      "alg_bytes_per_step": total_bytes, "launch_us_mean": launch_us},
         "cpu_baseline": cpu,
         "extra": extra,

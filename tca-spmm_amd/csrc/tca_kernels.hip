@@ -1381,9 +1381,6 @@ tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, in
     return TCA_OK;
 }
 
-namespace {
-}  // namespace
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------

@@ -49,8 +49,15 @@ BATCH_VARIANTS = {"spmm": {16: (16, 32, 64), 32: (32,)}, "spmm_t": {16: (16, 32,
 def choice_for(M, K, nnz, N, op="spmm", batch=False):
     """(rows_per_wave, tile_n) for an M x K matrix (op "spmm") or for the
     product with the transpose of a K x M matrix whose transpose is M x K
-    (op "spmm_t"), with nnz nonzeros and N dense columns.  batch=True keeps
-    to the variants the persistent batch kernel has."""
+    (op "spmm_t"), with nnz nonzeros and N dense columns.
+
+    batch=True: a variant the persistent batch kernel has.  There the
+    parallelism comes from the many items, and every work unit pays a fixed
+    claim + pipeline-start cost, so the widest tile (fewer, longer units) is
+    taken: 16 rows x 64 columns (measured faster than the per-call tuned
+    choices on the twelve-layer forward + backward step)."""
+    if batch:
+        return 16, 64 if N >= 64 else (32 if N >= 32 else 16)
     d = nnz / float(max(1, M * K))
     best, best_dist = None, None
     for e in _load()["entries"]:
@@ -60,13 +67,8 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False):
         if best is None or dist < best_dist:
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
-        for c in best.get("candidates", [best]):
-            if not batch or c["tile_n"] in BATCH_VARIANTS[op].get(c["rows_per_wave"], ()):
-                return c["rows_per_wave"], c["tile_n"]
-    rpw, tile = heuristic(M, K, nnz, N)
-    if batch and tile not in BATCH_VARIANTS[op].get(rpw, ()):
-        rpw, tile = 16, min(tile, 64)
-    return rpw, tile
+        return best["rows_per_wave"], best["tile_n"]
+    return heuristic(M, K, nnz, N)
 
 
 def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False):

@@ -1,10 +1,10 @@
 """Batched persistent launch over the twelve uneven layers (forward SpMM,
 transposed SpMM dX and SDDMM dW of every layer in ONE kernel): the outputs
-must match the per-call kernels to FP32 summation-order noise (the per-call
-launches split a group's chunks, or the SDDMM contraction, across waves and
-add the partial sums in a fixed order), match the FP64 oracle, and stay
-bit-identical across repeated HIP-graph replays (the launch resets its own
-work counter)."""
+must match the per-call kernels (SpMM to FP32 summation-order noise: the two
+launches split a group's chunks differently; SDDMM bit for bit), match the
+FP64 oracle, and repeat across HIP-graph replays (the launch resets its own
+work counter): the SDDMM bit for bit, the SpMM outputs to FP32
+summation-order noise (units split across workgroups add atomically)."""
 import numpy as np
 import pytest
 import torch
@@ -53,7 +53,8 @@ def test_batch_equals_per_call_and_oracle(cuda):
         Y = T.spmm(L["plan"], L["w"], L["X"], tile_n=L["tile"])
         dX = T.spmm(L["plan_t"], L["w"], L["G"], tile_n=L["tile_t"])
         dW = T.sddmm(L["plan"], L["G"], L["X"])
-        for got, ref in ((L["Y"], Y), (L["dX"], dX), (L["dW"], dW)):
+        assert torch.equal(L["dW"], dW)
+        for got, ref in ((L["Y"], Y), (L["dX"], dX)):
             assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
     for L in layers[::5]:
         W, X, G = L["case"].A, L["case"].B, L["case"].extra["G"]
@@ -88,7 +89,11 @@ def test_batch_graph_replays(cuda):
         g.replay()
         torch.cuda.synchronize()
         for L, (y, dx, dw) in zip(layers, ref):
-            assert torch.equal(L["Y"], y) and torch.equal(L["dX"], dx) and torch.equal(L["dW"], dw), rep
+            # units split across workgroups add atomically (order may vary between
+            # replays); the SDDMM is never split and must repeat bit for bit
+            assert torch.equal(L["dW"], dw), rep
+            for got, want in ((L["Y"], y), (L["dX"], dx)):
+                assert rel_l2(got.cpu().numpy(), want.cpu().numpy()) <= 1e-6, rep
 
 
 def test_batch_rejects_unsupported(cuda):
