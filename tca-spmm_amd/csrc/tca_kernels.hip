@@ -219,6 +219,18 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wait for every vector-memory load issued so far.  Placed between a software
+// pipeline's prologue and its loop: the compiler derives each wait in the loop
+// from the state merged over the loop's entry and its back edge, and an entry
+// with loads still in flight (issued in another order than in the steady
+// state) makes it wait for nearly the whole ring at the top of every iteration.
+// An entry with nothing in flight leaves the steady-state counts.  The first
+// chunks' loads are needed at once anyway, so the drain costs nothing.
+__device__ __forceinline__ void vm_drain()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt / lgkmcnt unconstrained
+}
+
 // Lane index that the compiler cannot treat as loop-invariant: inside the
 // persistent batch loop, values derived from the lane would otherwise be
 // hoisted out of the loop for every variant and stay live across all of them.
@@ -227,6 +239,22 @@ __device__ __forceinline__ int lane_id()
     int l = (int)threadIdx.x & 63;
     asm volatile("" : "+v"(l));
     return l;
+}
+
+// Global-address-space view of a pointer.  Pointers the batch kernel reads out
+// of its item descriptors are generic to the compiler, which would then emit
+// FLAT accesses; while a FLAT load is pending, every wait on the vector-memory
+// counter becomes a full drain (vmcnt(0)), which serialises the chunk pipeline.
+// Every device pointer here is global memory, so the cast is exact.
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T *gbl(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// Relaxed device-scope float add in global memory (hardware atomic, no CAS loop).
+__device__ __forceinline__ void atomic_add_f32(float *p, float v)
+{
+    [[clang::atomic(ignore_denormal_mode)]] { __hip_atomic_fetch_add(gbl(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 }
 
 struct SpmmParams {
@@ -394,7 +422,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
 #pragma unroll
             for (int j = 0; j < KPL; j++) ix.cols[j] = (int)buf_b32(r_cols, cb + 4u * j);
         }
-        ix.segb = p.seg_ptr[c * PPG + seg_lane];
+        ix.segb = gbl(p.seg_ptr)[c * PPG + seg_lane];
     };
     // Loads are issued unconditionally, results are only used at compute time:
     // masking a loaded value here would wait for it (and, the counter being in
@@ -461,8 +489,8 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             overflow[i] = st.e1[i] - st.e0[i] > kEP * 64;
             if (overflow[i])
                 for (int e = st.e0[i] + kEP * 64 + lane; e < st.e1[i]; e += 64) {
-                    const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_lds[e])) = vv;
+                    const E vv = INDIRECT ? gbl(avals)[gbl(p.entry_src)[e]] : gbl(avals)[e];
+                    *reinterpret_cast<E *>(lds_a + a_off(i, gbl(p.entry_lds)[e])) = vv;
                 }
         }
         wave_lds_sync();
@@ -511,6 +539,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
             load_index(min(c_lo + s + D, c_last), ix[s]);
         }
         load_vals(st[0]);
+        vm_drain();
         for (int c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
             for (int s = 0; s < D; s++) {
@@ -540,16 +569,16 @@ __device__ __forceinline__ void spmm_store_direct(const SpmmParams &p, int g, in
         for (int j = 0; j < 4; j++) {
             const int row = rbase + 16 * i + j;
             if (row >= p.n_rows) continue;
-            const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+            const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
             float *crow = p.C + (long)row * p.ldc;
 #pragma unroll
             for (int s = 0; s < NT / 16; s++) {
                 const int n = n0 + 16 * s + (lane & 15);
                 if (n < p.N) {
                     float v = acc[i][s][j] * rs;
-                    if (p.col_scale) v *= p.col_scale[n];
-                    if (atomic) unsafeAtomicAdd(crow + n, v);
-                    else crow[n] = v;
+                    if (p.col_scale) v *= gbl(p.col_scale)[n];
+                    if (atomic) atomic_add_f32(crow + n, v);
+                    else gbl(crow)[n] = v;
                 }
             }
         }
@@ -607,25 +636,26 @@ __device__ __forceinline__ void spmm_wg_unit(const SpmmParams &p, int g, int til
         }
         const int row = g * RPW + r, n = n0 + c;
         if (row >= p.n_rows || n >= p.N) continue;
-        const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+        const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
         v.x *= rs;
         v.y *= rs;
         v.z *= rs;
         v.w *= rs;
         if (p.col_scale) {
-            v.x *= p.col_scale[n];
-            v.y *= p.col_scale[n + 1];
-            v.z *= p.col_scale[n + 2];
-            v.w *= p.col_scale[n + 3];
+            const auto cs = gbl(p.col_scale);
+            v.x *= cs[n];
+            v.y *= cs[n + 1];
+            v.z *= cs[n + 2];
+            v.w *= cs[n + 3];
         }
         float *dst = p.C + (long)row * p.ldc + n;
         if (atomic) {
-            unsafeAtomicAdd(dst, v.x);
-            unsafeAtomicAdd(dst + 1, v.y);
-            unsafeAtomicAdd(dst + 2, v.z);
-            unsafeAtomicAdd(dst + 3, v.w);
+            atomic_add_f32(dst, v.x);
+            atomic_add_f32(dst + 1, v.y);
+            atomic_add_f32(dst + 2, v.z);
+            atomic_add_f32(dst + 3, v.w);
         } else {
-            *reinterpret_cast<float4 *>(dst) = v;
+            *gbl(reinterpret_cast<f32x4 *>(dst)) = f32x4{v.x, v.y, v.z, v.w};
         }
     }
 }
@@ -742,7 +772,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     auto load_index = [&](int c, Index &ix) {
 #pragma unroll
         for (int j = 0; j < L::PPT; j++) ix.col[j] = p.chunk_cols[(long)c * 32 + pk[j]];
-        ix.segb = p.seg_ptr[(long)c * PPG + w * PPW + seg_lane];
+        ix.segb = gbl(p.seg_ptr)[(long)c * PPG + w * PPW + seg_lane];
     };
     auto load_data = [&](const Index &ix, Stage &st) {
         st.eok = 0;
@@ -802,8 +832,8 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             overflow[i] = st.e1[i] - st.e0[i] > EP * 64;
             if (overflow[i])
                 for (int e = st.e0[i] + EP * 64 + lane; e < st.e1[i]; e += 64) {
-                    const E vv = INDIRECT ? avals[p.entry_src[e]] : avals[e];
-                    *reinterpret_cast<E *>(lds_a + a_off(i, p.entry_lds[e])) = vv;
+                    const E vv = INDIRECT ? gbl(avals)[gbl(p.entry_src)[e]] : gbl(avals)[e];
+                    *reinterpret_cast<E *>(lds_a + a_off(i, gbl(p.entry_lds)[e])) = vv;
                 }
         }
         wave_lds_sync();
@@ -844,6 +874,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             load_index(min(c_lo + s + D, c_last), ix[s]);
         }
         load_vals(st[0]);
+        vm_drain();
         for (int c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
             for (int s = 0; s < D; s++) {
@@ -869,16 +900,16 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         for (int j = 0; j < 4; j++) {
             const int row = rbase + 16 * i + j;
             if (row >= p.n_rows) continue;
-            const float rs = p.row_scale ? p.row_scale[row] : 1.0f;
+            const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
             float *crow = p.C + (long)row * p.ldc;
 #pragma unroll
             for (int s = 0; s < NSUB; s++) {
                 const int n = n0 + 16 * s + (lane & 15);
                 if (n < p.N) {
                     float v = acc[i][s][j] * rs;
-                    if (p.col_scale) v *= p.col_scale[n];
-                    if (p.split) unsafeAtomicAdd(crow + n, v);
-                    else crow[n] = v;
+                    if (p.col_scale) v *= gbl(p.col_scale)[n];
+                    if (p.split) atomic_add_f32(crow + n, v);
+                    else gbl(crow)[n] = v;
                 }
             }
         }
@@ -1001,7 +1032,7 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     using GB = RowBuf<AT, GT>;
     using XB = RowBuf<AT, AT>;
     const int w = threadIdx.x >> 6, lane = lane_id(), tid = w * 64 + lane;
-    const int nc = min(CB, p.grp_chunk_ptr[g + 1] - c0);
+    const int nc = min(CB, gbl(p.grp_chunk_ptr)[g + 1] - c0);
     const Rsrc r_g = make_rsrc(p.G, p.g_bytes);
     const Rsrc r_x = make_rsrc(p.X, p.x_bytes);
 
@@ -1013,7 +1044,7 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     for (int j = 0; j < L::XP; j++) {
         const int pc = tid + 256 * j, row = pc >> 2, q = pc & 3, b = row >> 5;
         xval[j] = pc < L::PIECES;
-        const int col = xval[j] && b < nc ? p.chunk_cols[(long)(c0 + b) * 32 + (row & 31)] : -1;
+        const int col = xval[j] && b < nc ? gbl(p.chunk_cols)[(long)(c0 + b) * 32 + (row & 31)] : -1;
         xsrc[j] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + 8u * q) * 2u : kOOB;
         xdst[j] = (row * kShPitch + 8 * q) * 2;
     }
@@ -1084,10 +1115,10 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
             for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][j];
         wave_lds_sync();
         const int c = c0 + b;
-        const int e0 = p.seg_ptr[c * plan_ppg + w], e1 = p.seg_ptr[c * plan_ppg + w + 1];
+        const int e0 = gbl(p.seg_ptr)[c * plan_ppg + w], e1 = gbl(p.seg_ptr)[c * plan_ppg + w + 1];
         for (int e = e0 + lane; e < e1; e += 64) {
-            const int slot = p.entry_slot[e];
-            p.out[p.dst ? p.dst[e] : e] = tile[(slot >> 5) * 33 + (slot & 31)];
+            const int slot = gbl(p.entry_slot)[e];
+            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(slot >> 5) * 33 + (slot & 31)];
         }
         wave_lds_sync();
     }
@@ -1214,11 +1245,12 @@ __global__ void warm_up_kernel() {}
 
 // ---------------------------------------------------------------------------
 // Batched persistent kernel: every workgroup (kBatchWaves waves) takes work
-// units from a shared counter until the list is exhausted, so one launch covers the
-// forward SpMM, the transposed SpMM and the SDDMM of many layers.  Units are
-// sorted heaviest first on the host.  counters[0] hands out units,
-// counters[1] counts finished workgroups; the last one resets both, so the
-// next launch (stream-ordered) starts from zero without a memset.
+// units until the list is exhausted, so one launch covers the forward SpMM,
+// the transposed SpMM and the SDDMM of many layers.  Units are sorted heaviest
+// first on the host; workgroup b starts with unit b, then claims from
+// counters[0] (units gridDim.x, gridDim.x + 1, ...).  counters[1] counts
+// finished workgroups; the last one resets both, so the next launch
+// (stream-ordered) starts from zero without a memset.
 // Variants (few and small, so that the kernel's register budget -- the maximum
 // over its cases -- keeps three waves on each SIMD): forward f16 x f16 on
 // 16-row plans with tile_n 16/32/64 (0..2) and on 32-row plans with tile_n 32
@@ -1297,11 +1329,17 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
 {
     __shared__ __attribute__((aligned(16))) uint8_t smem[kBatchLds];
     __shared__ int next_unit;
-    for (;;) {
-        if (threadIdx.x == 0) next_unit = atomicAdd(&counters[0], 1);
-        __syncthreads();
-        const int u = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
-        __syncthreads();                                            // all have read next_unit
+    // Workgroup b takes unit b first without touching the counter (every
+    // workgroup claiming at once would queue on the one atomic address); later
+    // units are claimed from the counter, which hands out gridDim.x + 0, 1, ...
+    for (bool first = true;; first = false) {
+        int u = blockIdx.x;
+        if (!first) {
+            if (threadIdx.x == 0) next_unit = (int)gridDim.x + atomicAdd(&counters[0], 1);
+            __syncthreads();
+            u = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
+            __syncthreads();                                  // all have read next_unit
+        }
         if (u >= n_units) break;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
