@@ -118,6 +118,7 @@ typedef struct tca_plan_info {
     int transposed;            /* 1 for a plan made by tca_plan_create_transpose */
     size_t device_bytes;
     double build_ms;
+    size_t dense_elems;        /* elements of the plan's panel-dense tiles (tca_pack_dense) */
 } tca_plan_info;
 
 tca_status tca_plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offset,
@@ -154,6 +155,19 @@ tca_status tca_plan_export(const tca_plan *plan, int32_t *grp_chunk_ptr, int32_t
 tca_status tca_pack_values(const tca_plan *plan, tca_dtype dtype, const void *d_csr, void *d_tiled, void *stream);
 tca_status tca_unpack_values(const tca_plan *plan, tca_dtype dtype, const void *d_tiled, void *d_csr, void *stream);
 
+/* Panel-dense tiles: every (chunk c, 16-row panel i) of the plan as a full
+ * 16 x 32 tile t = c*(rows_per_wave/16) + i of info.dense_elems / 512 tiles,
+ * element ((k/8)*16 + r)*8 + k%8 holding A[panel row r][chunk column k] (the
+ * MFMA operand order: lane r + 16*(k/8) reads 8 consecutive k), zero where
+ * the pattern has no entry.  Built from tiled values in the forward plan's
+ * order (for a transposed plan: the forward plan's tiled values, placed
+ * through its entry map).  F16/BF16 (1 KiB per tile) or E4M3 (512 B); d_dense
+ * 16-byte aligned.  No counterpart in the reference, whose kernel reads CSR
+ * values per nonzero (src/our_spmm.cu:290-346): the dense tiles trade a few
+ * more bytes (a tile is 1/16 of the B rows a chunk gathers at N = 256) for a
+ * chunk loop without per-entry work. */
+tca_status tca_pack_dense(const tca_plan *plan, tca_dtype dtype, const void *d_tiled, void *d_dense, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* SpMM: C[M x N] (f32, row-major, ldc) = A (plan + tiled values) * B.       */
 /* Replaces kernel_our_spmm_balanced (src/our_spmm.cu:290-346).              */
@@ -170,6 +184,13 @@ tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_til
                     tca_dtype b_dtype, const void *d_B, int ldb, int N,
                     float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
                     int tile_n, void *stream);
+/* tca_spmm with A as panel-dense tiles (tca_pack_dense) of a plan of 16, 32
+ * or 64 rows per group, forward or transposed; same arguments and results
+ * (bit for bit: the MFMAs see the same operands in the same order). */
+tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_dense,
+                          tca_dtype b_dtype, const void *d_B, int ldb, int N,
+                          float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
+                          int tile_n, void *stream);
 
 /* SDDMM on the plan's pattern: out_tiled[e] = sum_n G[row(e), n] * X[col(e), n]
  * (the dW of Y = W X with dY = G).  G: M x N row-major (ldg), X: K x N (ldx),
@@ -193,8 +214,9 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 /* Batched persistent launch over many layers (one kernel, capturable).      */
 /* SPMM items: F16 values; forward with an F16 B on 16-row plans (tile_n     */
 /* 16/32/64) or 32-row plans (tile_n 32); transposed with an F32 or F16 B on  */
-/* 16-row plans (tile_n 16/32/64).  SDDMM items: F16 X, F32 or F16 G.  Other  */
-/* combinations return TCA_ERR_UNSUPPORTED.                                  */
+/* 16-row plans (tile_n 16/32/64); panel-dense F16 A (a_dense) with an F16  */
+/* or F32 B on 16-row plans, forward or transposed (tile_n 16/32/64).  SDDMM */
+/* items: F16 X, F32 or F16 G.  Other combinations: TCA_ERR_UNSUPPORTED.     */
 /* ------------------------------------------------------------------------ */
 typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
 typedef struct tca_batch_item {
@@ -210,6 +232,7 @@ typedef struct tca_batch_item {
     int N;
     float *out;            /* SPMM: C (M x N, ldc);     SDDMM: out_tiled     */
     int ldc;
+    int a_dense;           /* SPMM: 1 = `a` holds panel-dense tiles (tca_pack_dense) */
 } tca_batch_item;
 typedef struct tca_batch tca_batch;
 tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch);

@@ -57,12 +57,16 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             if (it.N <= 0 || it.N % 8 || it.ldb < it.N || it.ldc < it.N || it.ldb % 8 || it.ldc % 4 || !it.b || !it.out ||
                 !aligned16(it.b) || !aligned16(it.out) || (t.nnz > 0 && !it.a))
                 return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SPMM item " + std::to_string(i) + " has bad shapes/buffers");
-            d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0);
+            d.variant = tca::batch_variant(TCA_OP_SPMM, t.rpw, tile, it.a_dtype, it.b_dtype, p->transposed != 0,
+                                           it.a_dense != 0);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
                                                      " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
                                                      "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 or f16 B "
-                                                     "transposed on rows_per_wave 16 with tile_n 16/32/64)");
+                                                     "transposed on rows_per_wave 16 with tile_n 16/32/64; panel-dense f16 A "
+                                                     "with f16 or f32 B on rows_per_wave 16 with tile_n 16/32/64)");
+            if (it.a_dense && tca::dense_bytes(t, it.a_dtype) >= (1L << 31))
+                return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " dense tiles reach 2 GiB");
             d.ldb = it.ldb;
             d.ldc = it.ldc;
             d.n_tiles = (it.N + tile - 1) / tile;
@@ -83,7 +87,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             if (it.N <= 0 || it.N % 32 || it.lda < it.N || it.ldb < it.N || it.lda % 8 || it.ldb % 8 || !it.a || !it.b ||
                 !aligned16(it.a) || !aligned16(it.b) || (t.nnz > 0 && !it.out))
                 return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM item " + std::to_string(i) + " has bad shapes/buffers");
-            d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false);
+            d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false, false);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) +
                                                      " (batched SDDMM: f16 X, f32 or f16 G)");

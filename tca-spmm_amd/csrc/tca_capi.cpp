@@ -262,6 +262,7 @@ extern "C" tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *inf
     info->transposed = plan->transposed;
     info->device_bytes = plan->d_bytes;
     info->build_ms = plan->build_ms;
+    info->dense_elems = (size_t)tca::dense_elems(t);
     return TCA_OK;
 }
 
@@ -302,9 +303,9 @@ extern "C" tca_status tca_unpack_values(const tca_plan *plan, tca_dtype dtype, c
     return tca::launch_gather(plan->d_entry_src, plan->host.nnz, eb, d_tiled, d_csr, true, static_cast<hipStream_t>(stream));
 }
 
-extern "C" tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled, tca_dtype b_dtype,
-                               const void *d_B, int ldb, int N, float *d_C, int ldc, const float *d_row_scale,
-                               const float *d_col_scale, int tile_n, void *stream)
+static tca_status spmm_entry(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled, tca_dtype b_dtype,
+                             const void *d_B, int ldb, int N, float *d_C, int ldc, const float *d_row_scale,
+                             const float *d_col_scale, int tile_n, bool dense, void *stream)
 {
     if (!plan) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: NULL plan");
     if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_spmm: host-only plan");
@@ -318,7 +319,35 @@ extern "C" tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const vo
     if (b_dtype == TCA_F32 && a_dtype == TCA_E4M3) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: quantise B to E4M3 first");
     if (b_dtype != a_dtype && b_dtype != TCA_F32) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: B dtype must equal A dtype or be F32");
     return tca::launch_spmm(plan, a_dtype, d_a_tiled, b_dtype, d_B, ldb, N, d_C, ldc, d_row_scale, d_col_scale, tile_n,
-                            static_cast<hipStream_t>(stream));
+                            dense, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled, tca_dtype b_dtype,
+                               const void *d_B, int ldb, int N, float *d_C, int ldc, const float *d_row_scale,
+                               const float *d_col_scale, int tile_n, void *stream)
+{
+    return spmm_entry(plan, a_dtype, d_a_tiled, b_dtype, d_B, ldb, N, d_C, ldc, d_row_scale, d_col_scale, tile_n, false,
+                      stream);
+}
+
+extern "C" tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_dense, tca_dtype b_dtype,
+                                     const void *d_B, int ldb, int N, float *d_C, int ldc, const float *d_row_scale,
+                                     const float *d_col_scale, int tile_n, void *stream)
+{
+    if (!aligned(d_a_dense, 16)) return fail(TCA_ERR_INVALID_ARG, "tca_spmm_dense: tiles must be 16-byte aligned");
+    return spmm_entry(plan, a_dtype, d_a_dense, b_dtype, d_B, ldb, N, d_C, ldc, d_row_scale, d_col_scale, tile_n, true,
+                      stream);
+}
+
+extern "C" tca_status tca_pack_dense(const tca_plan *plan, tca_dtype dtype, const void *d_tiled, void *d_dense,
+                                     void *stream)
+{
+    if (!plan || (plan->host.nnz > 0 && !d_tiled) || (plan->host.n_chunks > 0 && !d_dense))
+        return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: NULL argument");
+    if (!plan->d_base) return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: host-only plan");
+    if (dtype != TCA_F16 && dtype != TCA_BF16 && dtype != TCA_E4M3) return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: dtype");
+    if (!aligned(d_dense, 16)) return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: tiles must be 16-byte aligned");
+    return tca::launch_pack_dense(plan, dtype == TCA_E4M3 ? 1 : 2, d_tiled, d_dense, static_cast<hipStream_t>(stream));
 }
 
 extern "C" tca_status tca_sddmm(const tca_plan *plan, tca_dtype g_dtype, const void *d_G, int ldg, tca_dtype x_dtype,

@@ -82,7 +82,11 @@ namespace tca {
 // Kernel launchers (tca_kernels.hip).
 tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_dtype b_dtype, const void *B,
                        int ldb, int N, float *C, int ldc, const float *row_scale, const float *col_scale,
-                       int tile_n, hipStream_t stream);
+                       int tile_n, bool dense, hipStream_t stream);
+// Panel-dense tiles of a plan: one 16 x 32 tile per (chunk, 16-row panel).
+inline long dense_elems(const PlanTables &t) { return (long)t.n_chunks * (t.rpw / 16) * 512; }
+inline long dense_bytes(const PlanTables &t, tca_dtype dt) { return dense_elems(t) * (dt == TCA_E4M3 ? 1 : 2); }
+tca_status launch_pack_dense(const tca_plan *p, int elem_bytes, const void *tiled, void *dense, hipStream_t stream);
 tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype,
                         const void *X, int ldx, int N, float *out, hipStream_t stream);
 tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *src, void *dst, bool scatter,
@@ -128,7 +132,7 @@ struct ZeroDesc {              // an output that split units accumulate into
 };
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // -1 when the combination has no batched variant.
-int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed);
+int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);
 // resident one-wave workgroups of batch_kernel per compute unit
 int batch_blocks_per_cu();
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,

@@ -349,10 +349,15 @@ constexpr int kDepth = 2;
 //   the current chunk is transposed / scattered into LDS and fed to the MFMAs.
 // atomic: several waves share the output tile (chunk ranges split a group);
 // the caller has zeroed C and the wave adds its partial sums.
-template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+// DENSE: p.a holds panel-dense tiles (every (chunk, panel) as a full 16 x 32
+// tile in MFMA operand order, see pack_dense_kernel): the A operand octet of a
+// lane is one vector load issued with the chunk's B rows, and neither the
+// entry lists nor lds_a are touched.
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT, bool DENSE = false>
 __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, uint8_t *lds_b,
                                           uint8_t *lds_a, f32x4 (&acc)[RPW / 16][NT / 16])
 {
+    static_assert(!(DENSE && INDIRECT), "panel-dense tiles are packed in the plan's own order");
     constexpr int PPG = RPW / 16;
     constexpr int ES = Elt<AT>::ES;
     constexpr int NSUB = NT / 16;
@@ -371,6 +376,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     struct Index {
         int cols[KPL];
         int segb;                      // lane j <= PPG: seg_ptr[c*PPG + j]
+        int c;                         // the chunk (DENSE: its tiles' index)
     };
     using RL = RowLoad<AT, BT>;
     using RB = RowBuf<AT, BT>;
@@ -381,6 +387,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         uint32_t val[PPG][kEP];        // value bits, or entry_src when INDIRECT
         uint32_t vv[PPG][kEP];         // INDIRECT: value bits, loaded one chunk ahead
         int e0[PPG], e1[PPG];
+        Oct aop[PPG];                  // DENSE: this lane's A operand octet of each panel
     };
 
     const int lane = lane_id();
@@ -404,11 +411,14 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     const Rsrc r_lds = make_rsrc(p.entry_lds, nnz * 2u);
     const Rsrc r_val = make_rsrc(p.a, nnz * (uint32_t)ES);
     const Rsrc r_src = make_rsrc(p.entry_src, nnz * 4u);
+    const Rsrc r_dense = make_rsrc(p.a, (uint32_t)p.n_chunks * (uint32_t)(PPG * 64 * OCT));
     const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
     const uint32_t col_bytes = (uint32_t)ncol * RB::BES;
 
+    if constexpr (!DENSE) {
 #pragma unroll
-    for (int i = 0; i < PPG; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+        for (int i = 0; i < PPG; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+    }
 
     auto load_index = [&](int c, Index &ix) {
         const uint32_t cb = ((uint32_t)c * 32u + (uint32_t)k0) * 4u;
@@ -422,7 +432,8 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
 #pragma unroll
             for (int j = 0; j < KPL; j++) ix.cols[j] = (int)buf_b32(r_cols, cb + 4u * j);
         }
-        ix.segb = gbl(p.seg_ptr)[c * PPG + seg_lane];
+        ix.c = c;
+        if constexpr (!DENSE) ix.segb = gbl(p.seg_ptr)[c * PPG + seg_lane];
     };
     // Loads are issued unconditionally, results are only used at compute time:
     // masking a loaded value here would wait for it (and, the counter being in
@@ -433,6 +444,15 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int j = 0; j < KPL; j++) {
             const bool ok = ix.cols[j] >= 0 && col_ok;
             st.rows[j] = RB::load(r_b, ok ? (uint32_t)ix.cols[j] * ldb_bytes + col_bytes : kOOB);
+        }
+        if constexpr (DENSE) {
+#pragma unroll
+            for (int i = 0; i < PPG; i++) {
+                const uint32_t off = ((uint32_t)(ix.c * PPG + i) * 64u + (uint32_t)lane) * (uint32_t)OCT;
+                if constexpr (OCT == 16) st.aop[i] = buf_b128(r_dense, off);
+                else st.aop[i] = buf_b64(r_dense, off);
+            }
+            return;
         }
 #pragma unroll
         for (int i = 0; i < PPG; i++) {
@@ -460,7 +480,43 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         }
     };
     auto a_off = [&](int i, int x) { return (i * 512 + x) * ES; };   // x: entry_lds index
+    auto compute_dense = [&](const Stage &st) {
+        Oct rows[KPL];
+#pragma unroll
+        for (int j = 0; j < KPL; j++) rows[j] = RL::cvt(st.rows[j]);
+        if constexpr (TR) {
+#pragma unroll
+            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = rows[j];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int n = 8 * seg + e;
+                const int off = b_oct(n, k0 >> 3) * OCT + (k0 & 7) * ES;
+                *reinterpret_cast<typename Store<KPL * ES>::T *>(lds_b + off) = pack_k<KPL, ES>(rows, e);
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            Oct b;
+            if constexpr (TR) {
+                const int off = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
+                const uint2 lo = lds_read_tr16(lds_b, off);
+                const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+                b = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
+            }
+#pragma unroll
+            for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(st.aop[i], b, acc[i][s]);
+        }
+        wave_lds_sync();
+    };
     auto compute = [&](const Stage &st) {
+        if constexpr (DENSE) {
+            compute_dense(st);
+            return;
+        }
         E v[PPG][kEP];
 #pragma unroll
         for (int i = 0; i < PPG; i++)
@@ -602,7 +658,7 @@ struct SpmmLds {
 // slot of `slot` bytes; the W partial tiles are then summed in wave order
 // (deterministic) and stored as float4 rows, or added atomically (`atomic`:
 // other workgroups hold other chunk ranges of the group; C was zeroed).
-template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT, bool DENSE = false>
 __device__ __forceinline__ void spmm_wg_unit(const SpmmParams &p, int g, int tile, int c_lo, int c_hi, bool atomic,
                                              uint8_t *smem, int slot)
 {
@@ -612,7 +668,7 @@ __device__ __forceinline__ void spmm_wg_unit(const SpmmParams &p, int g, int til
     const int nch = c_hi - c_lo;
     const int lo = c_lo + (nch * w) / W, hi = c_lo + (nch * (w + 1)) / W;
     f32x4 acc[RPW / 16][NT / 16];
-    spmm_unit<RPW, NT, AT, BT, INDIRECT>(p, g, tile, lo, hi, mine, mine + L::B, acc);
+    spmm_unit<RPW, NT, AT, BT, INDIRECT, DENSE>(p, g, tile, lo, hi, mine, mine + L::B, acc);
 
     float *red = reinterpret_cast<float *>(mine);
 #pragma unroll
@@ -663,7 +719,7 @@ __device__ __forceinline__ void spmm_wg_unit(const SpmmParams &p, int g, int til
 // Standalone launch: workgroup -> (unit of the plan's unit table, N-tile) by
 // xcd_work, run by spmm_wg_unit.  Units of a group split across workgroups
 // (p.split) accumulate atomically into a zeroed C.
-template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+template <int RPW, int NT, tca_dtype AT, tca_dtype BT, bool INDIRECT, bool DENSE = false>
 __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel(SpmmParams p)
 {
     using L = SpmmLds<RPW, NT, AT>;
@@ -671,7 +727,7 @@ __global__ __launch_bounds__((64 * SpmmLds<RPW, NT, AT>::WMAX)) void spmm_kernel
     int ui, tile;
     if (!xcd_work(p, blockIdx.x, ui, tile)) return;
     const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
-    spmm_wg_unit<RPW, NT, AT, BT, INDIRECT>(p, u.x, tile, u.z, u.w, p.split != 0, smem, L::SLOT);
+    spmm_wg_unit<RPW, NT, AT, BT, INDIRECT, DENSE>(p, u.x, tile, u.z, u.w, p.split != 0, smem, L::SLOT);
 }
 
 // ---------------------------------------------------------------------------
@@ -918,25 +974,25 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 
 using SpmmFn = void (*)(SpmmParams);
 
-template <int RPW, tca_dtype AT, tca_dtype BT, bool IND>
+template <int RPW, tca_dtype AT, tca_dtype BT, bool IND, bool DN = false>
 SpmmFn pick_nt(int nt)
 {
     switch (nt) {
-    case 16: return spmm_kernel<RPW, 16, AT, BT, IND>;
-    case 32: return spmm_kernel<RPW, 32, AT, BT, IND>;
-    case 64: return spmm_kernel<RPW, 64, AT, BT, IND>;
-    case 128: return spmm_kernel<RPW, 128, AT, BT, IND>;
+    case 16: return spmm_kernel<RPW, 16, AT, BT, IND, DN>;
+    case 32: return spmm_kernel<RPW, 32, AT, BT, IND, DN>;
+    case 64: return spmm_kernel<RPW, 64, AT, BT, IND, DN>;
+    case 128: return spmm_kernel<RPW, 128, AT, BT, IND, DN>;
     }
     return nullptr;
 }
 
-template <tca_dtype AT, tca_dtype BT, bool IND>
+template <tca_dtype AT, tca_dtype BT, bool IND, bool DN = false>
 SpmmFn pick_rpw(int rpw, int nt)
 {
     switch (rpw) {
-    case 16: return pick_nt<16, AT, BT, IND>(nt);
-    case 32: return pick_nt<32, AT, BT, IND>(nt);
-    case 64: return pick_nt<64, AT, BT, IND>(nt);
+    case 16: return pick_nt<16, AT, BT, IND, DN>(nt);
+    case 32: return pick_nt<32, AT, BT, IND, DN>(nt);
+    case 64: return pick_nt<64, AT, BT, IND, DN>(nt);
     }
     return nullptr;
 }
@@ -976,6 +1032,17 @@ SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
     if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_rpw<TCA_E4M3, TCA_E4M3, IND>(rpw, nt);
     if (at == TCA_F16 && bt == TCA_F32) return pick_rpw<TCA_F16, TCA_F32, IND>(rpw, nt);
     if (at == TCA_BF16 && bt == TCA_F32) return pick_rpw<TCA_BF16, TCA_F32, IND>(rpw, nt);
+    return nullptr;
+}
+
+// Panel-dense A (plans of 16 / 32 / 64 rows per group, forward or transposed).
+SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt)
+{
+    if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
+    if (at == TCA_BF16 && bt == TCA_BF16) return pick_rpw<TCA_BF16, TCA_BF16, false, true>(rpw, nt);
+    if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_rpw<TCA_E4M3, TCA_E4M3, false, true>(rpw, nt);
+    if (at == TCA_F16 && bt == TCA_F32) return pick_rpw<TCA_F16, TCA_F32, false, true>(rpw, nt);
+    if (at == TCA_BF16 && bt == TCA_F32) return pick_rpw<TCA_BF16, TCA_F32, false, true>(rpw, nt);
     return nullptr;
 }
 
@@ -1134,6 +1201,36 @@ __global__ __launch_bounds__(256) void sddmm_shared_kernel(SddmmParams p)
 // ---------------------------------------------------------------------------
 // small elementwise kernels
 // ---------------------------------------------------------------------------
+// Panel-dense tiles: one wave per (chunk, panel) segment s = c*PPG + i.  The
+// segment's entries are scattered into a zeroed 16 x 32 tile in LDS at
+// element ((k/8)*16 + r)*8 + k%8 (entry slot = r << 5 | k), i.e. the octet of
+// MFMA operand lane r + 16*(k/8); the tile is then written out whole, one
+// octet per lane.  src holds tiled values; src_idx (a transposed plan's entry
+// map into the forward plan's tiled order) may be null.
+template <class T>
+__global__ __launch_bounds__(256) void pack_dense_kernel(long n_segs, const int32_t *__restrict__ seg_ptr,
+                                                         const uint16_t *__restrict__ entry_slot,
+                                                         const int32_t *__restrict__ src_idx, const T *__restrict__ src,
+                                                         T *__restrict__ dst)
+{
+    constexpr int OCT = 8 * (int)sizeof(T);
+    using O = typename Store<OCT>::T;
+    __shared__ __attribute__((aligned(16))) T tile[4][512];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long seg = (long)blockIdx.x * 4 + w;
+    if (seg >= n_segs) return;   // whole waves; no workgroup barrier below
+    T *t = tile[w];
+    reinterpret_cast<O *>(t)[lane] = O{};
+    wave_lds_sync();
+    const int e1 = seg_ptr[seg + 1];
+    for (int e = seg_ptr[seg] + lane; e < e1; e += 64) {
+        const int slot = entry_slot[e], r = slot >> 5, k = slot & 31;
+        t[((k >> 3) * 16 + r) * 8 + (k & 7)] = src[src_idx ? src_idx[e] : e];
+    }
+    wave_lds_sync();
+    reinterpret_cast<O *>(dst + seg * 512)[lane] = reinterpret_cast<const O *>(t)[lane];
+}
+
 template <class T>
 __global__ void gather_kernel(const int32_t *__restrict__ idx, int n, const T *__restrict__ src, T *__restrict__ dst,
                               bool scatter)
@@ -1354,6 +1451,12 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         case 9: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 10: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 11: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 12: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 13: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 14: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 15: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 16: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 17: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
         case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
@@ -1380,8 +1483,16 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 
 }  // namespace
 
-int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed)
+int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense)
 {
+    if (dense) {   // panel-dense tiles, forward or transposed plan alike
+        if (op != TCA_OP_SPMM || a_dtype != TCA_F16 || rpw != 16) return -1;
+        const int t = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
+        if (t < 0) return -1;
+        if (b_dtype == TCA_F16) return 12 + t;
+        if (b_dtype == TCA_F32) return 15 + t;
+        return -1;
+    }
     if (op == TCA_OP_SPMM) {
         if (a_dtype != TCA_F16) return -1;
         if (!transposed && b_dtype == TCA_F16) {
@@ -1499,12 +1610,16 @@ static tca_status spmm_units(const tca_plan *cp, int S, const int4 **units, int 
 
 tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_dtype b_dtype, const void *B, int ldb,
                        int N, float *C, int ldc, const float *row_scale, const float *col_scale, int tile_n,
-                       hipStream_t stream)
+                       bool dense, hipStream_t stream)
 {
     const PlanTables &t = p->host;
     if (tile_n == 0) tile_n = default_tile_n(p, N);
-    SpmmFn fn = p->transposed ? pick_spmm<true>(a_dtype, b_dtype, t.rpw, tile_n)
-                              : pick_spmm<false>(a_dtype, b_dtype, t.rpw, tile_n);
+    if (dense && t.rpw >= 128) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_dense: plans of 16, 32 or 64 rows per group");
+    if (dense && dense_bytes(t, a_dtype) >= (1L << 31))
+        return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_dense: the dense tiles reach 2 GiB (32-bit offsets)");
+    SpmmFn fn = dense           ? pick_spmm_dense(a_dtype, b_dtype, t.rpw, tile_n)
+                : p->transposed ? pick_spmm<true>(a_dtype, b_dtype, t.rpw, tile_n)
+                                : pick_spmm<false>(a_dtype, b_dtype, t.rpw, tile_n);
     if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: unsupported dtype pair / rows_per_wave / tile_n");
     SpmmParams prm;
     prm.grp_chunk_ptr = p->d_grp_chunk_ptr;
@@ -1670,6 +1785,25 @@ tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *
         break;
     default: return fail(TCA_ERR_INVALID_ARG, "gather: element size");
     }
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+tca_status launch_pack_dense(const tca_plan *p, int elem_bytes, const void *tiled, void *dense, hipStream_t stream)
+{
+    const long n_segs = (long)p->host.n_chunks * (p->host.rpw / 16);
+    if (n_segs <= 0) return TCA_OK;
+    const long blocks = (n_segs + 3) / 4;
+    if (blocks > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_pack_dense: too many tiles");
+    const int32_t *src_idx = p->transposed ? p->d_entry_src : nullptr;
+    if (elem_bytes == 2)
+        hipLaunchKernelGGL(pack_dense_kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, stream, n_segs, p->d_seg_ptr,
+                           p->d_entry_slot, src_idx, static_cast<const uint16_t *>(tiled), static_cast<uint16_t *>(dense));
+    else if (elem_bytes == 1)
+        hipLaunchKernelGGL(pack_dense_kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, stream, n_segs, p->d_seg_ptr,
+                           p->d_entry_slot, src_idx, static_cast<const uint8_t *>(tiled), static_cast<uint8_t *>(dense));
+    else
+        return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: element size");
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }

@@ -31,14 +31,14 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("n_rows", ctypes.c_int), ("n_cols", ctypes.c_int), ("nnz", ctypes.c_int),
                 ("rows_per_wave", ctypes.c_int), ("n_groups", ctypes.c_int), ("n_chunks", ctypes.c_int),
                 ("max_group_chunks", ctypes.c_int), ("transposed", ctypes.c_int),
-                ("device_bytes", ctypes.c_size_t), ("build_ms", ctypes.c_double)]
+                ("device_bytes", ctypes.c_size_t), ("build_ms", ctypes.c_double), ("dense_elems", ctypes.c_size_t)]
 
 
 class BatchItem(ctypes.Structure):
     _fields_ = [("op", ctypes.c_int), ("plan", ctypes.c_void_p), ("tile_n", ctypes.c_int),
                 ("a_dtype", ctypes.c_int), ("a", ctypes.c_void_p), ("lda", ctypes.c_int),
                 ("b_dtype", ctypes.c_int), ("b", ctypes.c_void_p), ("ldb", ctypes.c_int),
-                ("N", ctypes.c_int), ("out", ctypes.c_void_p), ("ldc", ctypes.c_int)]
+                ("N", ctypes.c_int), ("out", ctypes.c_void_p), ("ldc", ctypes.c_int), ("a_dense", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -80,6 +80,8 @@ SIGNATURES = {
     "tca_pack_values": (_i, [_vp, _i, _vp, _vp, _vp]),
     "tca_unpack_values": (_i, [_vp, _i, _vp, _vp, _vp]),
     "tca_spmm": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp]),
+    "tca_spmm_dense": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp]),
+    "tca_pack_dense": (_i, [_vp, _i, _vp, _vp, _vp]),
     "tca_sddmm": (_i, [_vp, _i, _vp, _i, _i, _vp, _i, _i, _vp, _vp]),
     "tca_quantize_rows_e4m3": (_i, [_i, _vp, _vp, _vp, _vp, _vp]),
     "tca_quantize_cols_e4m3": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _vp]),

@@ -17,8 +17,9 @@ from ._lib import OP_SDDMM, OP_SPMM, call
 from .core import _stream, tca_dtype
 
 
-def spmm_item(plan, a_tiled, B, out, tile_n=0):
-    return dict(op=OP_SPMM, plan=plan, a=a_tiled, b=B, out=out, tile_n=tile_n)
+def spmm_item(plan, a_tiled, B, out, tile_n=0, dense=False):
+    """dense=True: a_tiled holds the plan's panel-dense tiles (Plan.pack_dense)."""
+    return dict(op=OP_SPMM, plan=plan, a=a_tiled, b=B, out=out, tile_n=tile_n, dense=dense)
 
 
 def sddmm_item(plan, G, X, out):
@@ -45,6 +46,7 @@ class Batch:
             e.N = b.shape[1]
             e.out = out.data_ptr()
             e.ldc = out.stride(0) if out.dim() == 2 else 0
+            e.a_dense = 1 if it.get("dense") else 0
         h = ctypes.c_void_p()
         call("tca_batch_create", len(items), arr, ctypes.byref(h))
         self._h = h

@@ -74,6 +74,7 @@ class Plan:
         self.transposed = bool(info.transposed)
         self.device_bytes = info.device_bytes
         self.build_ms = info.build_ms
+        self.dense_elems = info.dense_elems
 
     @property
     def handle(self):
@@ -82,7 +83,8 @@ class Plan:
     def info(self):
         return dict(n_rows=self.n_rows, n_cols=self.n_cols, nnz=self.nnz, rows_per_wave=self.rows_per_wave,
                     n_groups=self.n_groups, n_chunks=self.n_chunks, max_group_chunks=self.max_group_chunks,
-                    transposed=self.transposed, device_bytes=self.device_bytes, build_ms=self.build_ms)
+                    transposed=self.transposed, device_bytes=self.device_bytes, build_ms=self.build_ms,
+                    dense_elems=self.dense_elems)
 
     def export(self):
         """Host copies of the tiling tables (numpy)."""
@@ -111,6 +113,19 @@ class Plan:
             out = torch.empty_like(values_csr)
         if self.nnz:
             call("tca_pack_values", self._h, tca_dtype(values_csr), _ptr(values_csr), _ptr(out), _stream(stream))
+        return out
+
+    def pack_dense(self, values_tiled, out=None, stream=None):
+        """Panel-dense tiles (dense_elems elements, same dtype) of tiled values:
+        every (chunk, 16-row panel) as a full 16 x 32 tile in MFMA operand
+        order, zeros off the pattern.  For a transposed plan pass the forward
+        plan's tiled values.  spmm(..., dense=True) takes the result."""
+        if out is None:
+            out = torch.empty(max(self.dense_elems, 1), dtype=values_tiled.dtype,
+                              device=values_tiled.device)[:self.dense_elems]
+        if self.dense_elems:
+            call("tca_pack_dense", self._h, tca_dtype(values_tiled), _ptr(values_tiled) if self.nnz else ctypes.c_void_p(0),
+                 _ptr(out), _stream(stream))
         return out
 
     def unpack(self, values_tiled, out=None, stream=None):
@@ -189,13 +204,15 @@ def csr_transpose_device(row_offset, col_idx, n_cols, stream=None):
     return col_offset, row_idx, perm
 
 
-def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, stream=None):
+def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, stream=None, dense=False):
     """C[M x N] (float32) = A (plan + tiled values) @ B.
 
     a_tiled: tiled values, float16 / bfloat16 / uint8 (E4M3 codes).  For a
     transposed plan pass the forward plan's tiled values.  B: K x N row-major,
     same dtype as A or float32 (converted in-kernel).  row_scale / col_scale
     (float32, may be None) scale the FP32 result per row / column.
+    dense=True: a_tiled holds the plan's panel-dense tiles (Plan.pack_dense);
+    the result is bit-identical to the tiled-values launch.
     """
     K, N = B.shape
     if K != plan.n_cols:
@@ -203,8 +220,10 @@ def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, s
     if out is None:
         out = torch.empty((plan.n_rows, N), dtype=torch.float32, device=B.device)
     at = tca_dtype(a_tiled) if a_tiled is not None else F16
-    call("tca_spmm", plan.handle, at, _ptr(a_tiled), tca_dtype(B), _ptr(B), B.stride(0), N, _ptr(out),
-         out.stride(0), _ptr(row_scale), _ptr(col_scale), tile_n, _stream(stream))
+    if dense and (a_tiled is None or a_tiled.numel() != plan.dense_elems):
+        raise ValueError(f"dense A must hold plan.dense_elems = {plan.dense_elems} elements")
+    call("tca_spmm_dense" if dense else "tca_spmm", plan.handle, at, _ptr(a_tiled), tca_dtype(B), _ptr(B), B.stride(0), N,
+         _ptr(out), out.stride(0), _ptr(row_scale), _ptr(col_scale), tile_n, _stream(stream))
     return out
 
 
