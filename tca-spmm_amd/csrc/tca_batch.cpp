@@ -184,19 +184,20 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
     const size_t bz = sizeof(tca::ZeroDesc) * std::max<size_t>(zero.size(), 1);
-    const size_t total = bi + bu + bz + 256;
+    const size_t bc = (bi + bu + bz + 255) / 256 * 256;   // counters start on a 256-byte line
+    const size_t total = bc + tca::kBatchCounterBytes;
     if (hipMalloc(&b->d_mem, total) != hipSuccess) { delete b; return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation"); }
     char *m = static_cast<char *>(b->d_mem);
     b->d_items = reinterpret_cast<tca::BatchItemDev *>(m);
     b->d_units = reinterpret_cast<tca::BatchUnitDev *>(m + bi);
     b->d_zero = reinterpret_cast<tca::ZeroDesc *>(m + bi + bu);
-    b->d_counters = reinterpret_cast<int *>(m + bi + bu + bz);
+    b->d_counters = reinterpret_cast<int *>(m + bc);
     if (hipMemcpy(b->d_items, dev.data(), bi, hipMemcpyHostToDevice) != hipSuccess ||
         (!sorted.empty() && hipMemcpy(b->d_units, sorted.data(), sizeof(tca::BatchUnitDev) * sorted.size(),
                                       hipMemcpyHostToDevice) != hipSuccess) ||
         (!zero.empty() && hipMemcpy(b->d_zero, zero.data(), sizeof(tca::ZeroDesc) * zero.size(),
                                     hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemset(b->d_counters, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        hipMemset(b->d_counters, 0, tca::kBatchCounterBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         tca_batch_destroy(b);
         return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
     }

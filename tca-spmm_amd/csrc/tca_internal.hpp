@@ -131,6 +131,13 @@ struct ZeroDesc {              // an output that split units accumulate into
     int32_t rows, N, ldc, pad;
 };
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
+// Work counters of a batch launch: the sorted unit list is dealt round-robin
+// to kBatchParts partitions (one per XCD under round-robin workgroup
+// dispatch), each with its own counter on its own 256-byte line, then one
+// count of finished workgroups.
+constexpr int kBatchParts = 8;
+constexpr int kCounterStride = 64;   // ints
+constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride * sizeof(int);
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);
 // resident one-wave workgroups of batch_kernel per compute unit

@@ -1344,10 +1344,10 @@ __global__ void warm_up_kernel() {}
 // Batched persistent kernel: every workgroup (kBatchWaves waves) takes work
 // units until the list is exhausted, so one launch covers the forward SpMM,
 // the transposed SpMM and the SDDMM of many layers.  Units are sorted heaviest
-// first on the host; workgroup b starts with unit b, then claims from
-// counters[0] (units gridDim.x, gridDim.x + 1, ...).  counters[1] counts
-// finished workgroups; the last one resets both, so the next launch
-// (stream-ordered) starts from zero without a memset.
+// first on the host and dealt to kBatchParts partitions, each with its own
+// counter (see the loop below); a last counter counts finished workgroups, and
+// the last workgroup resets them all, so the next launch (stream-ordered)
+// starts from zero without a memset.
 // Variants (few and small, so that the kernel's register budget -- the maximum
 // over its cases -- keeps three waves on each SIMD): forward f16 x f16 on
 // 16-row plans with tile_n 16/32/64 (0..2) and on 32-row plans with tile_n 32
@@ -1426,18 +1426,26 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
 {
     __shared__ __attribute__((aligned(16))) uint8_t smem[kBatchLds];
     __shared__ int next_unit;
-    // Workgroup b takes unit b first without touching the counter (every
-    // workgroup claiming at once would queue on the one atomic address); later
-    // units are claimed from the counter, which hands out gridDim.x + 0, 1, ...
+    // Partition x of P holds units x, x + P, x + 2P, ... (heaviest first) and
+    // is served by the workgroups b with b % P == x.  A workgroup's first unit
+    // is static (its rank r in the partition); later ones come from the
+    // partition's counter, which hands out ranks gx + 0, 1, ....  One atomic
+    // address per partition keeps the claims from queueing on a single line.
+    const int P = min(kBatchParts, (int)gridDim.x);
+    const int x = (int)blockIdx.x % P, r = (int)blockIdx.x / P;
+    const int gx = ((int)gridDim.x - x + P - 1) / P;
+    int *cnt = counters + x * kCounterStride;
     for (bool first = true;; first = false) {
-        int u = blockIdx.x;
+        int k = r;
         if (!first) {
-            if (threadIdx.x == 0) next_unit = (int)gridDim.x + atomicAdd(&counters[0], 1);
+            if (threadIdx.x == 0) next_unit = gx + atomicAdd(cnt, 1);
             __syncthreads();
-            u = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
+            k = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
             __syncthreads();                                  // all have read next_unit
         }
-        if (u >= n_units) break;
+        const long uu = (long)k * P + x;
+        if (uu >= n_units) break;
+        const int u = (int)uu;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
@@ -1465,9 +1473,9 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
     }
     if (threadIdx.x == 0) {
         __threadfence();
-        if (atomicAdd(&counters[1], 1) == (int)gridDim.x - 1) {
-            atomicExch(&counters[0], 0);
-            atomicExch(&counters[1], 0);
+        if (atomicAdd(&counters[kBatchParts * kCounterStride], 1) == (int)gridDim.x - 1) {
+            for (int q = 0; q < kBatchParts; q++) atomicExch(&counters[q * kCounterStride], 0);
+            atomicExch(&counters[kBatchParts * kCounterStride], 0);
         }
     }
 }
