@@ -4,15 +4,17 @@
 Workload (one "step"): the FFN-shaped GPU-forward suite -- CSR A of 2048x512
 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 (lognormal(0.5) row lengths,
 values N(0,0.02)), B = K x 256 N(0,1), FP16 inputs, FP32 accumulate and output:
-ten SpMMs per step, inputs resident in HBM, run as ONE persistent batched
+ten SpMMs per step, inputs resident in HBM (A as panel-dense 16 x 32 tiles of
+16-row plans, built once like the tiled values), run as ONE persistent batched
 launch (the ten products' work units drain through one grid), K launches back
 to back.  value = sum(2*nnz*N) / step time (TFLOP/s).
 
 Also reported: the HBM roofline of that launch (the ten products' algorithmic
 bytes / its measured duration vs 8 TB/s), the ten tuned per-call launches
-(graph and eager), per-matrix warm and cold launch times, rocSPARSE and
-hipBLASLt on the same inputs, and the CPU oracle (FP32-accumulate C port of
-naive_gemm restricted to nonzeros) timed on the host cores.
+(dense tiles: graph and eager; tiled values: graph), per-matrix warm and cold
+launch times, rocSPARSE and hipBLASLt on the same inputs, and the CPU oracle
+(FP32-accumulate C port of naive_gemm restricted to nonzeros) timed on the
+host cores.
 Prints one JSON line on stdout.
 """
 import argparse
@@ -90,8 +92,12 @@ def time_eager(torch, fn, iters, backlog=False):
 
 
 def bench_fwd_bwd(torch, np, T, O, dev):
-    """Twelve uneven layers, forward + dX + dW: per-call path (36 launches) vs
-    one persistent launch replayed from a HIP graph."""
+    """Twelve uneven layers, forward + dX + dW: per-call path (36 launches,
+    tuned tiled-values launches and tuned panel-dense launches) vs one
+    persistent launch (panel-dense forward and dX, SDDMM dW) replayed back to
+    back and from a HIP graph.  Weights are fixed over the timed steps, so
+    their tiled values and dense tiles are built before timing; `repack_ms`
+    is what rebuilding them after a weight update costs per step."""
     from tca_spmm import workloads
     from tca_spmm.workloads import alg_bytes_dw, alg_bytes_dx, alg_bytes_fwd
     layers, items, flops, nbytes = [], [], 0, 0
@@ -100,61 +106,68 @@ def bench_fwd_bwd(torch, np, T, O, dev):
         w_vals = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
         Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
         Gd = torch.from_numpy(G).to(dev)
-        L = dict(X=Xd, G=Gd,
+        L = dict(X=Xd, G=Gd, w_vals=w_vals,
                  Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
                  dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
                  dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
-        # per-call path: the tuned best launch for each product
-        rpw, L["tile"] = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N)
-        rpw_t, L["tile_t"] = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t")
-        L["plan"] = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-        L["w"] = L["plan"].pack(w_vals)
-        fwd16 = L["plan"] if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
-        L["plan_t"] = fwd16.transpose(rpw_t)
-        L["w16"] = L["w"] if rpw == 16 else fwd16.pack(w_vals)
-        # batched path: the best of the variants the persistent kernel has
-        rpw_b, tile_b = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True)
-        rpw_bt, tile_bt = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True)
-        pb = fwd16 if rpw_b == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw_b)
-        wb = L["w16"] if rpw_b == 16 else pb.pack(w_vals)
-        pbt = L["plan_t"] if rpw_bt == rpw_t else fwd16.transpose(rpw_bt)
-        L["keep"] = (pb, wb, pbt)
-        layers.append(L)
-        items += [T.spmm_item(pb, wb, Xd, L["Y"], tile_b), T.spmm_item(pbt, L["w16"], Gd, L["dX"], tile_bt),
-                  T.sddmm_item(fwd16, Gd, Xd, L["dW"])]
+        fwd16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+        L["w16"] = fwd16.pack(w_vals)
         L["sddmm_plan"] = fwd16
+        # per-call paths: the tuned launch of each product, tiled values and dense tiles
+        for lay, dense in (("t", False), ("d", True)):
+            rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, dense=dense)
+            rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", dense=dense)
+            plan = fwd16 if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+            w = L["w16"] if rpw == 16 else plan.pack(w_vals)
+            plan_t = fwd16.transpose(rpw_t)
+            L[lay] = dict(plan=plan, plan_t=plan_t, tile=tile, tile_t=tile_t, dense=dense,
+                          a=plan.pack_dense(w) if dense else w,
+                          at=plan_t.pack_dense(L["w16"]) if dense else L["w16"])
+        # batched path: 16-row plans, 64-column tiles, panel-dense forward and dX
+        tile_b = 64 if case.N >= 64 else (32 if case.N >= 32 else 16)
+        pbt = fwd16.transpose(16)
+        L["b"] = dict(plan=fwd16, plan_t=pbt, a=fwd16.pack_dense(L["w16"]), at=pbt.pack_dense(L["w16"]))
+        items += [T.spmm_item(fwd16, L["b"]["a"], Xd, L["Y"], tile_b, dense=True),
+                  T.spmm_item(pbt, L["b"]["at"], Gd, L["dX"], tile_b, dense=True),
+                  T.sddmm_item(fwd16, Gd, Xd, L["dW"])]
         flops += 3 * 2 * W.nnz * case.N
         nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
                    + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, case.N))
+        layers.append(L)
 
-    def per_call():
-        for L in layers:
-            T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
-            T.spmm(L["plan_t"], L["w16"], L["G"], out=L["dX"], tile_n=L["tile_t"])
-            T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
-    def only(op):
+    def per_call(lay, op=None):
         def run():
             for L in layers:
-                if op == "fwd":
-                    T.spmm(L["plan"], L["w"], L["X"], out=L["Y"], tile_n=L["tile"])
-                elif op == "dx":
-                    T.spmm(L["plan_t"], L["w16"], L["G"], out=L["dX"], tile_n=L["tile_t"])
-                else:
+                c = L[lay]
+                if op in (None, "fwd"):
+                    T.spmm(c["plan"], c["a"], L["X"], out=L["Y"], tile_n=c["tile"], dense=c["dense"])
+                if op in (None, "dx"):
+                    T.spmm(c["plan_t"], c["at"], L["G"], out=L["dX"], tile_n=c["tile_t"], dense=c["dense"])
+                if op in (None, "dw"):
                     T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
         return run
+
+    def repack():
+        for L in layers:
+            L["sddmm_plan"].pack(L["w_vals"], out=L["w16"])
+            L["b"]["plan"].pack_dense(L["w16"], out=L["b"]["a"])
+            L["b"]["plan_t"].pack_dense(L["w16"], out=L["b"]["at"])
+
     bt = T.Batch(items)
-    plain_eager = time_eager(torch, per_call, 20)
-    plain_graph = time_graph(torch, per_call, 20)
-    # the fused step is one batch launch (plus the fill kernel zeroing the
-    # outputs of split units): launched back to back, and replayed from a graph
+    plain_eager = time_eager(torch, per_call("t"), 20)
+    plain_graph = time_graph(torch, per_call("t"), 20)
+    dense_graph = time_graph(torch, per_call("d"), 20)
     fused = time_eager(torch, bt.run, 50, backlog=True)
     fused_graph = time_graph(torch, bt.run, 50)
-    per_op = {op: time_graph(torch, only(op), 20) for op in ("fwd", "dx", "dw")}
-    return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph, "fused_ms": fused,
-            "fused_graph_ms": fused_graph,
-            "speedup_vs_eager": plain_eager / fused, "speedup_vs_graph": plain_graph / fused,
+    repack_ms = time_graph(torch, repack, 20)
+    per_op = {lay: {op: time_graph(torch, per_call(lay, op), 20) for op in ("fwd", "dx", "dw")} for lay in ("t", "d")}
+    best_plain = min(plain_graph, dense_graph)
+    return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph,
+            "dense_per_call_graph_ms": dense_graph, "fused_ms": fused, "fused_graph_ms": fused_graph,
+            "speedup_vs_graph": best_plain / fused, "speedup_vs_tiled_graph": plain_graph / fused,
+            "speedup_vs_eager": plain_eager / fused, "repack_ms": repack_ms,
             "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "per_call_graph_ms_by_op": per_op,
+            "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
             "launches_plain": 3 * len(layers), "units": bt.n_units, "workgroups": bt.n_workgroups}
 
 
@@ -181,15 +194,21 @@ def bench_precision(torch, np, T, O, dev):
     va = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(dev)
     vb = torch.from_numpy(case.B.view(np.int16)).view(torch.float16).to(dev)
     C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
+    rpw_d, tile_d = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=True)
+    plan_d = plan if rpw_d == rpw else T.Plan(A.n_rows, A.n_cols, rp, ci, rpw_d)
     for name, cast in (("f16", torch.float16), ("bf16", torch.bfloat16)):
         ta, tb = plan.pack(va.to(cast)), vb.to(cast)
+        da = plan_d.pack_dense(plan_d.pack(va.to(cast)))
+        ms_d = time_graph(torch, lambda: T.spmm(plan_d, da, tb, out=C, tile_n=tile_d, dense=True), 50)
+        Cd = C.cpu().numpy()
         ms = time_graph(torch, lambda: T.spmm(plan, ta, tb, out=C, tile_n=tile), 50)
         Cn = C.cpu().numpy()
         rq = O.spmm_f64(A.n_rows, N, rp, ci, plan.unpack(ta).float().cpu().numpy().astype(np.float64),
                         tb.float().cpu().numpy().astype(np.float64))
         e_all, m_all = err(Cn, ref)
         e_q, m_q = err(Cn, rq)
-        out[name] = {"us": ms * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+        out[name] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+                     "dense_tflops": 2 * A.nnz * N / (ms_d * 1e-3) / 1e12, "dense_rel_l2_vs_fp64": err(Cd, ref)[0],
                      "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
                      "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q}
@@ -197,6 +216,9 @@ def bench_precision(torch, np, T, O, dev):
     qa, sa = T.quantize_rows_e4m3(rpd, va)
     qb, sb = T.quantize_cols_e4m3(vb)
     ta = plan.pack(qa)
+    da = plan_d.pack_dense(plan_d.pack(qa))
+    ms_d = time_graph(torch, lambda: T.spmm(plan_d, da, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile_d, dense=True), 50)
+    Cd = C.cpu().numpy()
     ms = time_graph(torch, lambda: T.spmm(plan, ta, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile), 50)
     Cn = C.cpu().numpy()
     rq = O.spmm_f64(A.n_rows, N, rp, ci, O.e4m3_to_f32(qa.cpu().numpy()).astype(np.float64),
@@ -205,12 +227,14 @@ def bench_precision(torch, np, T, O, dev):
     e_all, m_all = err(Cn, ref)
     e_q, m_q = err(Cn, rq)
     qms = time_graph(torch, lambda: T.quantize_cols_e4m3(vb), 20)
-    out["e4m3"] = {"us": ms * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+    out["e4m3"] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
+                   "dense_tflops": 2 * A.nnz * N / (ms_d * 1e-3) / 1e12, "dense_rel_l2_vs_fp64": err(Cd, ref)[0],
                    "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N, 1, 1, True) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
                    "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q,
                    "quantize_cols_us": qms * 1e3}
-    out["config"] = {"rows_per_wave": rpw, "tile_n": tile, "nnz": A.nnz, "N": N}
+    out["config"] = {"rows_per_wave": rpw, "tile_n": tile, "dense_rows_per_wave": rpw_d, "dense_tile_n": tile_d,
+                     "nnz": A.nnz, "N": N}
     return out
 
 
@@ -262,34 +286,38 @@ def main():
     entries = []
     for case in suite:
         A = case.A
-        rpw, tile_n = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N)
-        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
-        tiled = plan.pack(f16(A.values))
-        B = f16(case.B)
-        C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
-        entries.append(dict(case=case, plan=plan, tiled=tiled, B=B, C=C, rpw=rpw, tile_n=tile_n,
-                            flops=2 * A.nnz * N, bytes=alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N)))
+        vals = f16(A.values)
+        e = dict(case=case, B=f16(case.B), C=torch.empty((A.n_rows, N), dtype=torch.float32, device=dev),
+                 flops=2 * A.nnz * N, bytes=alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N))
+        # per-call launches: each product's tuned (rows_per_wave, tile_n), with A
+        # as tiled values ("t") and as panel-dense tiles ("d")
+        for lay, dense in (("t", False), ("d", True)):
+            rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=dense)
+            plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
+            a = plan.pack(vals)
+            e[lay] = dict(plan=plan, a=plan.pack_dense(a) if dense else a, tile=tile, rpw=rpw, dense=dense)
+        # the step's batch item: 16-row plan, 64-column tiles, panel-dense tiles
+        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
+        pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
+        e["b"] = dict(plan=pb, a=pb.pack_dense(pb.pack(vals)), tile=tile_b, rpw=rpw_b)
+        entries.append(e)
     total_flops = sum(e["flops"] for e in entries)
     total_bytes = sum(e["bytes"] for e in entries)
 
-    def per_call():
-        for e in entries:
-            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+    def launch(e, lay):
+        c = e[lay]
+        T.spmm(c["plan"], c["a"], e["B"], out=e["C"], tile_n=c["tile"], dense=c["dense"])
+
+    def per_call(lay):
+        def run():
+            for e in entries:
+                launch(e, lay)
+        return run
 
     # The step: the ten products as ONE persistent batched launch (their work
-    # units drain through one grid), the fastest way found to run them; the
-    # batch uses 16-row plans with 64-column tiles (autotune.choice_for(batch=True)).
-    bitems = []
-    for e in entries:
-        A = e["case"].A
-        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
-        if rpw_b == e["rpw"]:
-            bitems.append(T.spmm_item(e["plan"], e["tiled"], e["B"], e["C"], tile_b))
-        else:
-            pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
-            e["batch_keep"] = (pb, pb.pack(f16(A.values)))
-            bitems.append(T.spmm_item(pb, e["batch_keep"][1], e["B"], e["C"], tile_b))
-    batch = T.Batch(bitems)
+    # units drain through one grid), the fastest way found to run them.
+    batch = T.Batch([T.spmm_item(e["b"]["plan"], e["b"]["a"], e["B"], e["C"], e["b"]["tile"], dense=True)
+                     for e in entries])
 
     def step():
         batch.run()
@@ -302,7 +330,7 @@ def main():
         A = e["case"].A
         refs.append(O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B))
     worst = 0.0
-    for run in (step, per_call):
+    for run in (step, per_call("t"), per_call("d")):
         for e in entries:
             e["C"].fill_(float("nan"))
         run()
@@ -342,41 +370,28 @@ def main():
     achieved = total_bytes / (ms_per_step * 1e-3) / 1e9
     graph_ms = time_graph(torch, step, K)
 
-    # the ten tuned per-call launches, captured in one graph (one stream) and eager
-    per_call_graph_ms = time_graph(torch, per_call, K)
-    for _ in range(3):
-        per_call()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(K):
-        per_call()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    eager_ms = e0.elapsed_time(e1) / K
+    # the ten tuned per-call launches, captured in one graph (one stream) and
+    # eager, with dense tiles and with tiled values
+    per_call_graph_ms = time_graph(torch, per_call("d"), K)
+    eager_ms = time_eager(torch, per_call("d"), K)
+    tiled_graph_ms = time_graph(torch, per_call("t"), K)
 
     extra = {"graph_replay_ms_per_step": graph_ms, "wall_s_timed": wall,
              "batch_units": batch.n_units, "batch_workgroups": batch.n_workgroups,
              "per_call_graph_ms_per_step": per_call_graph_ms, "per_call_eager_ms_per_step": eager_ms,
+             "per_call_tiled_graph_ms_per_step": tiled_graph_ms,
              "speedup_vs_per_call_graph": per_call_graph_ms / ms_per_step,
-             "configs": [{"name": e["case"].A.name, "nnz": e["case"].A.nnz, "rows_per_wave": e["rpw"],
-                          "tile_n": e["tile_n"]} for e in entries]}
+             "configs": [{"name": e["case"].A.name, "nnz": e["case"].A.nnz,
+                          "per_call_dense": [e["d"]["rpw"], e["d"]["tile"]], "per_call_tiled": [e["t"]["rpw"], e["t"]["tile"]],
+                          "batch": [e["b"]["rpw"], e["b"]["tile"]]} for e in entries]}
 
-    # per-matrix times (eager events around each launch after a backlog)
+    # per-matrix times of the tuned per-call launch (dense tiles; tiled values
+    # beside it): 50 back-to-back launches after a queue backlog
     per = []
     for e in entries:
-        s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        for _ in range(3):
-            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
-        if hasattr(torch.cuda, "_sleep"):
-            torch.cuda._sleep(2_000_000)
-        s.record(stream)
-        for _ in range(50):
-            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
-        t.record(stream)
-        torch.cuda.synchronize()
-        us = s.elapsed_time(t) * 1e3 / 50
-        per.append({"name": e["case"].A.name, "us": us, "tflops": e["flops"] / (us * 1e-6) / 1e12,
+        us = time_eager(torch, lambda: launch(e, "d"), 50, backlog=True) * 1e3
+        us_t = time_eager(torch, lambda: launch(e, "t"), 50, backlog=True) * 1e3
+        per.append({"name": e["case"].A.name, "us": us, "tiled_us": us_t, "tflops": e["flops"] / (us * 1e-6) / 1e12,
                     "hbm_frac": e["bytes"] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS})
     # cold: a 512 MiB write between launches evicts L2 and the Infinity Cache, so
     # every operand comes from HBM (event pair around the launch alone)
@@ -387,7 +402,7 @@ def main():
             flush.fill_(1)
             s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record(stream)
-            T.spmm(e["plan"], e["tiled"], e["B"], out=e["C"], tile_n=e["tile_n"])
+            launch(e, "d")
             t.record(stream)
             torch.cuda.synchronize()
             cold.append(s.elapsed_time(t) * 1e3)
@@ -468,7 +483,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain_gpu: CSR 2048x512 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 "
                                "(lognormal(0.5) rows), B Kx256, FP16 in, FP32 accumulate/out; 10 SpMMs per step "
-                               "as one persistent batched launch",
+                               "as one persistent batched launch, A as panel-dense 16x32 tiles",
                    "N": N, "matrices": len(entries), "nnz_total": int(sum(e["case"].A.nnz for e in entries)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -8,6 +8,8 @@ behind a sleep so host enqueue cost stays off the clock) and returns them
 fastest first.  Results recorded with `record()` go to tuning_table.json next
 to this file; `choice_for()` looks up the nearest recorded shape (same M, K,
 N class, closest density) and falls back to a density rule when none exists.
+Entries are kept per A layout: tiled values, or panel-dense tiles
+(`layout="dense"`, plans of 16/32/64 rows only).
 """
 import json
 import math
@@ -46,10 +48,11 @@ def heuristic(M, K, nnz, N):
 BATCH_VARIANTS = {"spmm": {16: (16, 32, 64), 32: (32,)}, "spmm_t": {16: (16, 32, 64)}}
 
 
-def choice_for(M, K, nnz, N, op="spmm", batch=False):
+def choice_for(M, K, nnz, N, op="spmm", batch=False, dense=False):
     """(rows_per_wave, tile_n) for an M x K matrix (op "spmm") or for the
     product with the transpose of a K x M matrix whose transpose is M x K
-    (op "spmm_t"), with nnz nonzeros and N dense columns.
+    (op "spmm_t"), with nnz nonzeros and N dense columns.  dense=True: for A
+    as panel-dense tiles.
 
     batch=True: a variant the persistent batch kernel has.  There the
     parallelism comes from the many items, and every work unit pays a fixed
@@ -59,23 +62,26 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False):
     if batch:
         return 16, 64 if N >= 64 else (32 if N >= 32 else 16)
     d = nnz / float(max(1, M * K))
+    layout = "dense" if dense else "tiled"
     best, best_dist = None, None
     for e in _load()["entries"]:
-        if e.get("op", "spmm") != op or e["M"] != M or e["K"] != K:
+        if e.get("op", "spmm") != op or e["M"] != M or e["K"] != K or e.get("layout", "tiled") != layout:
             continue
         dist = abs(math.log(max(e["density"], 1e-6)) - math.log(max(d, 1e-6))) + abs(math.log(e["N"] / N))
         if best is None or dist < best_dist:
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
         return best["rows_per_wave"], best["tile_n"]
-    return heuristic(M, K, nnz, N)
+    rpw, tile = heuristic(M, K, nnz, N)
+    return (min(rpw, 64), tile) if dense else (rpw, tile)
 
 
-def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False):
+def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False, dense=False):
     """Time each (rows_per_wave, tile_n) on the device.  A: workloads.SparseMatrix
     (fp16 values), B_dev: K x N device tensor (for transposed=True: the
-    M x N float32 gradient, and the product is A^T B).  Returns a list of
-    dicts sorted by time (us per launch)."""
+    M x N float32 gradient, and the product is A^T B).  dense=True times A as
+    panel-dense tiles (16/32/64-row candidates).  Returns a list of dicts
+    sorted by time (us per launch)."""
     import numpy as np
     import torch
 
@@ -85,6 +91,8 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", tr
         values_dev = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(device)
     N = B_dev.shape[1]
     cands = candidates or ALL_CANDIDATES
+    if dense:
+        cands = [c for c in cands if c[0] in RPW_CANDIDATES]
     out = torch.empty((A.n_cols if transposed else A.n_rows, N), dtype=torch.float32, device=device)
     stream = torch.cuda.current_stream()
     res = []
@@ -94,20 +102,22 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", tr
     for rpw, tile in cands:
         if rpw not in plans:
             if transposed:
-                plans[rpw] = (fwd.transpose(rpw), fwd_tiled)
+                pt = fwd.transpose(rpw)
+                plans[rpw] = (pt, pt.pack_dense(fwd_tiled) if dense else fwd_tiled)
             else:
                 p = Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
-                plans[rpw] = (p, p.pack(values_dev))
+                t = p.pack(values_dev)
+                plans[rpw] = (p, p.pack_dense(t) if dense else t)
         p, tiled = plans[rpw]
         for _ in range(3):
-            spmm(p, tiled, B_dev, out=out, tile_n=tile)
+            spmm(p, tiled, B_dev, out=out, tile_n=tile, dense=dense)
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         if hasattr(torch.cuda, "_sleep"):
             torch.cuda._sleep(1_000_000)
         s.record(stream)
         for _ in range(iters):
-            spmm(p, tiled, B_dev, out=out, tile_n=tile)
+            spmm(p, tiled, B_dev, out=out, tile_n=tile, dense=dense)
         e.record(stream)
         torch.cuda.synchronize()
         res.append({"rows_per_wave": rpw, "tile_n": tile, "us": s.elapsed_time(e) * 1e3 / iters})
@@ -115,16 +125,16 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", tr
     return res
 
 
-def record(A, N, results, op="spmm", path=None):
-    """Store the fastest candidate for this shape in the tuning table."""
+def record(A, N, results, op="spmm", path=None, layout="tiled"):
+    """Store the fastest candidate for this shape (and A layout) in the tuning table."""
     tab = _load()
     d = A.nnz / float(A.n_rows * A.n_cols)
     best = results[0]
     M, K = (A.n_cols, A.n_rows) if op == "spmm_t" else (A.n_rows, A.n_cols)
     tab["entries"] = [e for e in tab["entries"]
-                      if not (e.get("op", "spmm") == op and e["M"] == M and e["K"] == K
-                              and e["N"] == N and abs(e["density"] - d) < 1e-3)]
-    tab["entries"].append({"op": op, "M": M, "K": K, "N": N, "density": round(d, 5),
+                      if not (e.get("op", "spmm") == op and e["M"] == M and e["K"] == K and e["N"] == N
+                              and abs(e["density"] - d) < 1e-3 and e.get("layout", "tiled") == layout)]
+    tab["entries"].append({"op": op, "layout": layout, "M": M, "K": K, "N": N, "density": round(d, 5),
                            "rows_per_wave": best["rows_per_wave"], "tile_n": best["tile_n"], "us": best["us"],
                            "candidates": results})
     with open(path or TABLE_PATH, "w") as f:
@@ -147,6 +157,7 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--workloads", default="plain_gpu,batch12,precision,sparse_linear")
     ap.add_argument("--iters", type=int, default=30)
+    ap.add_argument("--layouts", default="tiled,dense")
     args = ap.parse_args(argv)
     global _table
     _table = {"entries": []}
@@ -167,16 +178,18 @@ def main(argv=None):
     if "sparse_linear" in sel:
         c = workloads.sparse_linear()
         jobs.append((c.A, c.B, c.extra["G"]))
-    for A, B, G in jobs:
-        r = tune(A, f16(B), iters=args.iters)
-        record(A, B.shape[1], r, op="spmm", path=args.out)
-        print(f"spmm   {A.name:40s} N={B.shape[1]:5d} best rpw={r[0]['rows_per_wave']} tile={r[0]['tile_n']} "
-              f"{r[0]['us']:.1f} us (worst {r[-1]['us']:.1f})", flush=True)
-        if G is not None:
-            rt = tune(A, torch.from_numpy(G).to(dev), iters=args.iters, transposed=True)
-            record(A, G.shape[1], rt, op="spmm_t", path=args.out)
-            print(f"spmm_t {A.name:40s} N={G.shape[1]:5d} best rpw={rt[0]['rows_per_wave']} tile={rt[0]['tile_n']} "
-                  f"{rt[0]['us']:.1f} us (worst {rt[-1]['us']:.1f})", flush=True)
+    for layout in args.layouts.split(","):
+        dense = layout == "dense"
+        for A, B, G in jobs:
+            r = tune(A, f16(B), iters=args.iters, dense=dense)
+            record(A, B.shape[1], r, op="spmm", path=args.out, layout=layout)
+            print(f"{layout:5s} spmm   {A.name:40s} N={B.shape[1]:5d} best rpw={r[0]['rows_per_wave']} "
+                  f"tile={r[0]['tile_n']} {r[0]['us']:.1f} us (worst {r[-1]['us']:.1f})", flush=True)
+            if G is not None:
+                rt = tune(A, torch.from_numpy(G).to(dev), iters=args.iters, transposed=True, dense=dense)
+                record(A, G.shape[1], rt, op="spmm_t", path=args.out, layout=layout)
+                print(f"{layout:5s} spmm_t {A.name:40s} N={G.shape[1]:5d} best rpw={rt[0]['rows_per_wave']} "
+                      f"tile={rt[0]['tile_n']} {rt[0]['us']:.1f} us (worst {rt[-1]['us']:.1f})", flush=True)
 
 
 if __name__ == "__main__":
