@@ -78,8 +78,11 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 const int nch = t.grp_chunk_ptr[g + 1] - t.grp_chunk_ptr[g];
                 for (int tt = 0; tt < d.n_tiles; tt++) {
                     units.push_back({i, g, tt, t.grp_chunk_ptr[g], t.grp_chunk_ptr[g + 1], 0, 0, 0});
-                    // ~ staging + MFMAs per chunk; an FP32 B gathers twice the bytes
-                    cost.push_back(nch * (2.0 + ppg * tile / 16.0 * (it.b_dtype == TCA_F32 ? 1.5 : 1.0)) + 4.0);
+                    // estimated ns: unit start + per chunk (index + B rows + MFMAs; an
+                    // FP32 B gathers twice the bytes; tiled values add the entry work)
+                    const double chunk = (100.0 + (it.b_dtype == TCA_F32 ? 280.0 : 150.0) * ppg * tile / 64.0) *
+                                         (it.a_dense ? 1.0 : 1.6);
+                    cost.push_back(1500.0 + nch * chunk);
                 }
             }
         } else if (it.op == TCA_OP_SDDMM) {
@@ -110,14 +113,15 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.ldc = it.lda;   // G
             d.a_bytes = tca::operand_bytes(t.n_rows, it.N, it.lda, it.a_dtype == TCA_F32 ? 4 : 2);
             d.b_bytes = tca::operand_bytes(t.n_cols, it.N, it.ldb, 2);
-            if (d.a_bytes == 0 || d.b_bytes == 0)
+            if (d.a_bytes == 0 || d.b_bytes == 0 || (long)t.nnz * 4 >= (1L << 31))
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) + " reaches 2 GiB");
             for (int gi = 0; gi < ta.n_groups; gi++) {
                 const int g = ta.work_order[gi];
                 for (int c = ta.grp_chunk_ptr[g]; c < ta.grp_chunk_ptr[g + 1]; c += tca::kBatchSddmmCB) {
                     const int nc = std::min(tca::kBatchSddmmCB, ta.grp_chunk_ptr[g + 1] - c);
                     units.push_back({i, g, c, 0, 0, 0, 0, 0});
-                    cost.push_back(nc * 2.0 * (tca::kSddmmRows / 16) * (it.N / 32.0) / 4.0 + 4.0);
+                    // estimated ns: unit start and sampling + per 32-wide contraction step
+                    cost.push_back(4000.0 + 700.0 * (it.N / 32.0) * nc / tca::kBatchSddmmCB);
                 }
             }
         } else {

@@ -133,10 +133,10 @@ struct ZeroDesc {              // an output that split units accumulate into
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // Work counters of a batch launch: the sorted unit list is dealt round-robin
 // to kBatchParts partitions (one per XCD under round-robin workgroup
-// dispatch), each with its own counter on its own 256-byte line, then one
-// count of finished workgroups.
+// dispatch), each with its own counter 4 KiB from the next (distinct memory
+// channels), then one count of finished workgroups.
 constexpr int kBatchParts = 8;
-constexpr int kCounterStride = 64;   // ints
+constexpr int kCounterStride = 1024;   // ints
 constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride * sizeof(int);
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);

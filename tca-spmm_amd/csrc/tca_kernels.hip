@@ -1066,6 +1066,7 @@ struct SddmmParams {
     const int32_t *unit_chunk;   // per workgroup: first chunk
     uint32_t g_bytes, x_bytes;   // bytes of G and X from the first to the last used element
     const int32_t *dst;          // out index of each entry (null: the entry's own index)
+    uint32_t nnz;                // entries of the tiling (entry_slot, dst)
 };
 
 // SDDMM with shared X rows.  The plan here has 64-row groups: a workgroup of 4
@@ -1090,6 +1091,11 @@ struct ShLds {
     static constexpr int XP = (PIECES + 255) / 256;             // X pieces per thread and step
 };
 
+// Entries per lane and chunk whose sampling indices are fetched before the
+// contraction (covers 128 entries of a 16 x 32 panel tile; denser tiles
+// fetch the rest while sampling).
+constexpr int kShEP = 2;
+
 template <int CB, tca_dtype AT, tca_dtype GT>
 __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, int c0, uint8_t *smem)
 {
@@ -1102,6 +1108,9 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     const int nc = min(CB, gbl(p.grp_chunk_ptr)[g + 1] - c0);
     const Rsrc r_g = make_rsrc(p.G, p.g_bytes);
     const Rsrc r_x = make_rsrc(p.X, p.x_bytes);
+    const Rsrc r_slot = make_rsrc(p.entry_slot, p.nnz * 2u);
+    const Rsrc r_dst = make_rsrc(p.dst, p.nnz * 4u);
+    const int plan_ppg = p.rpw >> 4;
 
     // this thread's X pieces: tile row (chunk b = row/32), 8-n octet q
     uint32_t xsrc[L::XP];
@@ -1120,6 +1129,24 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     // B operand of (chunk b, half h): X tile row 32b + 16h + lane%16, octet lane/16
     const int xread = ((lane & 15) * kShPitch + 8 * (lane >> 4)) * 2;
 
+    // sampling indices of this wave's panel in each chunk, issued now so that
+    // they arrive during the contraction (out-of-range entries read as zero
+    // and are skipped when sampling)
+    int e0[CB], e1[CB];
+    uint32_t slot[CB][kShEP], dsti[CB][kShEP];
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+        const int c = c0 + min(b, nc - 1);
+        e0[b] = gbl(p.seg_ptr)[c * plan_ppg + w];
+        e1[b] = gbl(p.seg_ptr)[c * plan_ppg + w + 1];
+#pragma unroll
+        for (int k = 0; k < kShEP; k++) {
+            const uint32_t e = (uint32_t)(e0[b] + k * 64 + lane);
+            slot[b][k] = buf_b16(r_slot, e * 2u);
+            dsti[b][k] = p.dst ? buf_b32(r_dst, e * 4u) : e;
+        }
+    }
+
     f32x4 acc[CB][2];
 #pragma unroll
     for (int b = 0; b < CB; b++)
@@ -1127,24 +1154,27 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
         for (int h = 0; h < 2; h++) acc[b][h] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int steps = p.N >> 5;
-    typename XB::Raw xr[L::XP];
+    // X pieces in a two-step register ring (xr[t & 1] holds step t's pieces),
+    // G rows two steps ahead
+    typename XB::Raw xr[2][L::XP];
     typename GB::Raw gr[2];
-    auto load_x = [&](int s) {
+    auto load_x = [&](int slot_, int t) {
 #pragma unroll
-        for (int j = 0; j < L::XP; j++) xr[j] = XB::load(r_x, xsrc[j], (uint32_t)(32 * s) * 2u);
+        for (int j = 0; j < L::XP; j++) xr[slot_][j] = XB::load(r_x, xsrc[j], (uint32_t)(32 * t) * 2u);
     };
-    auto put_x = [&](int s) {
-        uint8_t *buf = smem + (s & 1) * L::XT;
+    auto put_x = [&](int slot_, int t) {
+        uint8_t *buf = smem + (t & 1) * L::XT;
 #pragma unroll
         for (int j = 0; j < L::XP; j++)
-            if (xval[j]) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[j];
+            if (xval[j]) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[slot_][j];
     };
-    // prologue: X of step 0 in LDS, X of step 1 and G of steps 0, 1 in registers
-    load_x(0);
+    // prologue: X of step 0 in LDS; X of steps 1, 2 and G of steps 0, 1 in registers
+    load_x(0, 0);
     gr[0] = GB::load(r_g, gsrc, 0);
     gr[1] = GB::load(r_g, gsrc, (uint32_t)(32 * min(1, steps - 1)) * GB::BES);
-    put_x(0);
-    load_x(min(1, steps - 1));
+    put_x(0, 0);
+    load_x(1, min(1, steps - 1));
+    load_x(0, min(2, steps - 1));
     __syncthreads();
     for (int s0 = 0; s0 < steps; s0 += 2) {
 #pragma unroll
@@ -1164,15 +1194,14 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
                     }
                 }
                 gr[k] = GB::load(r_g, gsrc, (uint32_t)(32 * min(s + 2, steps - 1)) * GB::BES);
-                put_x(s + 1);                       // X of step s+1 (a clamped re-read past the end)
-                load_x(min(s + 2, steps - 1));
+                put_x(1 - k, s + 1);                      // X of step s+1, loaded two steps ago
+                load_x(1 - k, min(s + 3, steps - 1));     // (clamped re-reads past the end)
                 __syncthreads();
             }
         }
     }
     // sample this wave's panel: stage each chunk's 16x32 tile in the wave's area
     float *tile = reinterpret_cast<float *>(smem + 2 * L::XT + w * L::STAGE);
-    const int plan_ppg = p.rpw >> 4;
 #pragma unroll
     for (int b = 0; b < CB; b++) {
         if (b >= nc) break;
@@ -1181,11 +1210,16 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
 #pragma unroll
             for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][j];
         wave_lds_sync();
-        const int c = c0 + b;
-        const int e0 = gbl(p.seg_ptr)[c * plan_ppg + w], e1 = gbl(p.seg_ptr)[c * plan_ppg + w + 1];
-        for (int e = e0 + lane; e < e1; e += 64) {
-            const int slot = gbl(p.entry_slot)[e];
-            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(slot >> 5) * 33 + (slot & 31)];
+#pragma unroll
+        for (int k = 0; k < kShEP; k++) {
+            if (e0[b] + k * 64 + lane < e1[b]) {
+                const uint32_t sl = slot[b][k];
+                gbl(p.out)[dsti[b][k]] = tile[(sl >> 5) * 33 + (sl & 31)];
+            }
+        }
+        for (int e = e0[b] + kShEP * 64 + lane; e < e1[b]; e += 64) {
+            const int sl = gbl(p.entry_slot)[e];
+            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(sl >> 5) * 33 + (sl & 31)];
         }
         wave_lds_sync();
     }
@@ -1407,6 +1441,7 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     p.unit_chunk = nullptr;
     p.g_bytes = d.a_bytes;
     p.x_bytes = d.b_bytes;
+    p.nnz = (uint32_t)d.nnz;
     p.dst = d.dst;
     return p;
 }
@@ -1471,8 +1506,12 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         }
         __syncthreads();  // LDS is reused by the next unit
     }
+    // No fence before the count: every claim of this workgroup has returned
+    // (its value was used), so the last workgroup's reset cannot overtake one;
+    // the outputs need no ordering against the counters (the next launch is
+    // stream-ordered).  A device-scope fence here would write back the XCD's
+    // L2 once per exiting workgroup and stall the units still running.
     if (threadIdx.x == 0) {
-        __threadfence();
         if (atomicAdd(&counters[kBatchParts * kCounterStride], 1) == (int)gridDim.x - 1) {
             for (int q = 0; q < kBatchParts; q++) atomicExch(&counters[q * kCounterStride], 0);
             atomicExch(&counters[kBatchParts * kCounterStride], 0);
@@ -1763,8 +1802,9 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     prm.dst = dst;
     prm.g_bytes = operand_bytes(t.n_rows, N, ldg, g_dtype == TCA_F32 ? 4 : 2);
     prm.x_bytes = operand_bytes(t.n_cols, N, ldx, 2);
-    if (prm.g_bytes == 0 || prm.x_bytes == 0)
-        return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: G or X reaches 2 GiB (32-bit offsets)");
+    prm.nnz = (uint32_t)t.nnz;
+    if (prm.g_bytes == 0 || prm.x_bytes == 0 || (long)t.nnz * 4 >= (1L << 31))
+        return fail(TCA_ERR_UNSUPPORTED, "tca_sddmm: G, X or the entry tables reach 2 GiB (32-bit offsets)");
     SddmmFn fn = cb == 4 ? pick_sddmm<4>(g_dtype, x_dtype) : cb == 2 ? pick_sddmm<2>(g_dtype, x_dtype)
                                                                    : pick_sddmm<1>(g_dtype, x_dtype);
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(256), 0, stream, prm);
