@@ -10,6 +10,7 @@ using tca::fail;
 
 struct tca_batch {
     int n_items = 0, n_units = 0, grid = 0, device = 0, n_zero = 0;
+    bool dense_only = false;   // every SpMM item has panel-dense tiles: the leaner kernel
     void *d_mem = nullptr;  // items | units | zero list | counters
     tca::BatchItemDev *d_items = nullptr;
     tca::BatchUnitDev *d_units = nullptr;
@@ -134,7 +135,10 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     int dev_id = 0, cus = 0;
     TCA_HIP_TRY(hipGetDevice(&dev_id));
     TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
-    const int workers = std::max(cus, 1) * tca::batch_blocks_per_cu();
+    bool dense_only = true;
+    for (int i = 0; i < n_items; i++)
+        if (items[i].op == TCA_OP_SPMM && !items[i].a_dense) dense_only = false;
+    const int workers = std::max(cus, 1) * tca::batch_blocks_per_cu(dense_only);
     double total_cost = 0.0;
     for (double c : cost) total_cost += c;
     const double share = total_cost / workers;
@@ -184,6 +188,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     b->n_units = (int)sorted.size();
     b->n_zero = (int)zero.size();
     b->device = dev_id;
+    b->dense_only = dense_only;
     b->grid = std::max(1, std::min(b->n_units, workers));
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
@@ -213,7 +218,7 @@ extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
 {
     if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
     return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->grid,
-                             batch->d_zero, batch->n_zero, static_cast<hipStream_t>(stream));
+                             batch->d_zero, batch->n_zero, batch->dense_only, static_cast<hipStream_t>(stream));
 }
 
 extern "C" tca_status tca_batch_destroy(tca_batch *batch)

@@ -141,8 +141,8 @@ constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);
 // resident one-wave workgroups of batch_kernel per compute unit
-int batch_blocks_per_cu();
+int batch_blocks_per_cu(bool dense_only);
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        const ZeroDesc *zero, int n_zero, hipStream_t stream);
+                        const ZeroDesc *zero, int n_zero, bool dense_only, hipStream_t stream);
 
 }  // namespace tca

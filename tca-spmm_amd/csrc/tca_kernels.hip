@@ -1455,6 +1455,10 @@ constexpr int kBatchSlot = SpmmLds<16, 64, TCA_F16>::SLOT > SpmmLds<32, 32, TCA_
 constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES ? kBatchWaves * kBatchSlot
                                                                                  : ShLds<kBatchSddmmCB>::BYTES;
 
+// DENSE_ONLY: the SpMM cases of panel-dense items only (and the SDDMM); the
+// register budget is then that of the lean dense loop, which keeps four
+// workgroups (16 waves) on each CU instead of three.
+template <bool DENSE_ONLY>
 __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                                  const BatchUnitDev *__restrict__ units, int n_units,
                                                                  int *counters)
@@ -1484,16 +1488,16 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
-        case 0: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 1: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 2: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 3: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 4: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 7: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 8: spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 9: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 10: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 11: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 0: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 1: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 2: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 3: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 4: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 7: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 8: if constexpr (!DENSE_ONLY) spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 9: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 10: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 11: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 12: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 13: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 14: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
@@ -1557,22 +1561,27 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
     return -1;
 }
 
-int batch_blocks_per_cu()
+int batch_blocks_per_cu(bool dense_only)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel, 64 * kBatchWaves, 0) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dense_only ? batch_kernel<true> : batch_kernel<false>,
+                                                     64 * kBatchWaves, 0) != hipSuccess || n < 1)
+        n = 1;
     return n;
 }
 
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        const ZeroDesc *zero, int n_zero, hipStream_t stream)
+                        const ZeroDesc *zero, int n_zero, bool dense_only, hipStream_t stream)
 {
     if (n_zero > 0) {
         hipLaunchKernelGGL(zero_list_kernel, dim3(256, n_zero), dim3(256), 0, stream, zero);
         TCA_HIP_TRY(hipGetLastError());
     }
     if (n_units <= 0) return TCA_OK;
-    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
+    if (dense_only)
+        hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
+    else
+        hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
