@@ -5,7 +5,7 @@ Workload (one "step"): the FFN-shaped GPU-forward suite -- CSR A of 2048x512
 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 (lognormal(0.5) row lengths,
 values N(0,0.02)), B = K x 256 N(0,1), FP16 inputs, FP32 accumulate and output:
 ten SpMMs per step, inputs resident in HBM (A as panel-dense 16 x 32 tiles of
-16-row plans, built once like the tiled values), run as ONE persistent batched
+32-row plans, built once like the tiled values), run as ONE persistent batched
 launch (the ten products' work units drain through one grid), K launches back
 to back.  value = sum(2*nnz*N) / step time (TFLOP/s).
 
@@ -123,12 +123,15 @@ def bench_fwd_bwd(torch, np, T, O, dev):
             L[lay] = dict(plan=plan, plan_t=plan_t, tile=tile, tile_t=tile_t, dense=dense,
                           a=plan.pack_dense(w) if dense else w,
                           at=plan_t.pack_dense(L["w16"]) if dense else L["w16"])
-        # batched path: 16-row plans, 64-column tiles, panel-dense forward and dX
-        tile_b = 64 if case.N >= 64 else (32 if case.N >= 32 else 16)
-        pbt = fwd16.transpose(16)
-        L["b"] = dict(plan=fwd16, plan_t=pbt, a=fwd16.pack_dense(L["w16"]), at=pbt.pack_dense(L["w16"]))
-        items += [T.spmm_item(fwd16, L["b"]["a"], Xd, L["Y"], tile_b, dense=True),
-                  T.spmm_item(pbt, L["b"]["at"], Gd, L["dX"], tile_b, dense=True),
+        # batched path: panel-dense forward and dX on the batch's choices, SDDMM dW
+        rpw_b, tile_b = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True, dense=True)
+        rpw_bt, tile_bt = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True, dense=True)
+        pb = fwd16 if rpw_b == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw_b)
+        wb = L["w16"] if rpw_b == 16 else pb.pack(w_vals)
+        pbt = fwd16.transpose(rpw_bt)
+        L["b"] = dict(plan=pb, plan_t=pbt, wb=wb, a=pb.pack_dense(wb), at=pbt.pack_dense(L["w16"]))
+        items += [T.spmm_item(pb, L["b"]["a"], Xd, L["Y"], tile_b, dense=True),
+                  T.spmm_item(pbt, L["b"]["at"], Gd, L["dX"], tile_bt, dense=True),
                   T.sddmm_item(fwd16, Gd, Xd, L["dW"])]
         flops += 3 * 2 * W.nnz * case.N
         nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
@@ -150,7 +153,9 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     def repack():
         for L in layers:
             L["sddmm_plan"].pack(L["w_vals"], out=L["w16"])
-            L["b"]["plan"].pack_dense(L["w16"], out=L["b"]["a"])
+            if L["b"]["wb"] is not L["w16"]:
+                L["b"]["plan"].pack(L["w_vals"], out=L["b"]["wb"])
+            L["b"]["plan"].pack_dense(L["b"]["wb"], out=L["b"]["a"])
             L["b"]["plan_t"].pack_dense(L["w16"], out=L["b"]["at"])
 
     bt = T.Batch(items)
@@ -296,8 +301,8 @@ def main():
             plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
             a = plan.pack(vals)
             e[lay] = dict(plan=plan, a=plan.pack_dense(a) if dense else a, tile=tile, rpw=rpw, dense=dense)
-        # the step's batch item: 16-row plan, 64-column tiles, panel-dense tiles
-        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True)
+        # the step's batch item: panel-dense tiles of a 32-row plan, 64-column tiles
+        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True, dense=True)
         pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
         e["b"] = dict(plan=pb, a=pb.pack_dense(pb.pack(vals)), tile=tile_b, rpw=rpw_b)
         entries.append(e)

@@ -65,7 +65,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                                                      " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
                                                      "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 or f16 B "
                                                      "transposed on rows_per_wave 16 with tile_n 16/32/64; panel-dense f16 A "
-                                                     "with f16 or f32 B on rows_per_wave 16 with tile_n 16/32/64)");
+                                                     "with f16 B on rows_per_wave 16 (tile_n 16/32/64), 32 (32/64) "
+                                                     "or 64 (32), or f32 B on rows_per_wave 16 (16/32/64) or 32 (32))");
             if (it.a_dense && tca::dense_bytes(t, it.a_dtype) >= (1L << 31))
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " dense tiles reach 2 GiB");
             d.ldb = it.ldb;

@@ -1383,11 +1383,15 @@ __global__ void warm_up_kernel() {}
 // the last workgroup resets them all, so the next launch (stream-ordered)
 // starts from zero without a memset.
 // Variants (few and small, so that the kernel's register budget -- the maximum
-// over its cases -- keeps three waves on each SIMD): forward f16 x f16 on
-// 16-row plans with tile_n 16/32/64 (0..2) and on 32-row plans with tile_n 32
-// (8); transposed f16 x f32 on 16-row plans with tile_n 16/32/64 (3, 4, 7) and
-// f16 x f16 (9..11); SDDMM with an f32 (5) or f16 (6) G on the plan's 64-row
-// SDDMM tiling.
+// over its cases -- keeps three waves on each SIMD, four in the dense-only
+// instantiation): tiled values, forward f16 x f16 on 16-row plans with tile_n
+// 16/32/64 (0..2) and on 32-row plans with tile_n 32 (8); transposed f16 x f32
+// on 16-row plans with tile_n 16/32/64 (3, 4, 7) and f16 x f16 (9..11);
+// panel-dense f16 A (forward or transposed) with an f16 B on 16-row plans with
+// tile_n 16/32/64 (12..14), 32-row plans with tile_n 32/64 (18, 19) and 64-row
+// plans with tile_n 32 (20), with an f32 B on 16-row plans with tile_n 16/32/64
+// (15..17) and 32-row plans with tile_n 32 (21); SDDMM with an f32 (5) or f16
+// (6) G on the plan's 64-row SDDMM tiling.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1449,9 +1453,9 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
 // The batch kernel's workgroup: kBatchWaves waves share every unit (SpMM:
 // chunk range split + LDS reduction; SDDMM: contraction split over N).
 constexpr int kBatchWaves = 4;
-constexpr int kBatchSlot = SpmmLds<16, 64, TCA_F16>::SLOT > SpmmLds<32, 32, TCA_F16>::SLOT   // largest SpMM
-                               ? SpmmLds<16, 64, TCA_F16>::SLOT                           // variant's per-wave LDS
-                               : SpmmLds<32, 32, TCA_F16>::SLOT;
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kBatchSlot = cmax(cmax(SpmmLds<16, 64, TCA_F16>::SLOT, SpmmLds<32, 32, TCA_F16>::SLOT),   // largest SpMM
+                                cmax(SpmmLds<32, 64, TCA_F16>::SLOT, SpmmLds<64, 32, TCA_F16>::SLOT));  // variant's per-wave LDS
 constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES ? kBatchWaves * kBatchSlot
                                                                                  : ShLds<kBatchSddmmCB>::BYTES;
 
@@ -1504,6 +1508,10 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         case 15: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 16: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 17: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 18: spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 19: spmm_wg_unit<32, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 20: spmm_wg_unit<64, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 21: spmm_wg_unit<32, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
         case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
@@ -1537,11 +1545,17 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense)
 {
     if (dense) {   // panel-dense tiles, forward or transposed plan alike
-        if (op != TCA_OP_SPMM || a_dtype != TCA_F16 || rpw != 16) return -1;
-        const int t = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
-        if (t < 0) return -1;
-        if (b_dtype == TCA_F16) return 12 + t;
-        if (b_dtype == TCA_F32) return 15 + t;
+        if (op != TCA_OP_SPMM || a_dtype != TCA_F16 || (b_dtype != TCA_F16 && b_dtype != TCA_F32)) return -1;
+        const int f32 = b_dtype == TCA_F32;
+        if (rpw == 16) {
+            const int t = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
+            return t < 0 ? -1 : (f32 ? 15 : 12) + t;
+        }
+        // (F32 B on wider panels only at rows_per_wave 32 x tile_n 32: the others
+        // would push the dense kernel past four workgroups per CU)
+        if (rpw == 32 && tile_n == 32) return f32 ? 21 : 18;
+        if (!f32 && rpw == 32 && tile_n == 64) return 19;
+        if (!f32 && rpw == 64 && tile_n == 32) return 20;
         return -1;
     }
     if (op == TCA_OP_SPMM) {

@@ -165,3 +165,29 @@ def test_dense_rejections(cuda):
     with pytest.raises(T.TcaError) as e:
         T.spmm(wide, wide.pack_dense(wide.pack(dev_f16(A.values))), dev_f16(case.B), dense=True)
     assert e.value.name == "TCA_ERR_UNSUPPORTED"
+
+
+@pytest.mark.parametrize("rpw,tile,g_f32", [(32, 32, False), (32, 64, False), (64, 32, False), (32, 32, True)])
+def test_dense_batch_wider_plans(cuda, rpw, tile, g_f32):
+    """Dense batch variants on 32- and 64-row plans (forward with an F16 B,
+    transposed with an F16 or F32 gradient) against per-call dense launches."""
+    keep, items, refs = [], [], []
+    for case in list(workloads.batch12(seed=0))[:4] + list(workloads.batch12(seed=0))[8:10]:
+        W, X, G = case.A, case.B, case.extra["G"]
+        fwd16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+        w16 = fwd16.pack(dev_f16(W.values))
+        if g_f32:                                  # transposed product, F32 gradient
+            plan = fwd16.transpose(rpw)
+            a, B = plan.pack_dense(w16), dev(G)
+        else:                                      # forward product, F16 activations
+            plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+            a, B = plan.pack_dense(plan.pack(dev_f16(W.values))), dev_f16(X)
+        out = torch.empty((plan.n_rows, case.N), dtype=torch.float32, device=cuda)
+        keep.append((fwd16, plan, a, B, out))
+        items.append(T.spmm_item(plan, a, B, out, tile, dense=True))
+        refs.append(T.spmm(plan, a, B, tile_n=tile, dense=True))
+    bt = T.Batch(items)
+    bt.run()
+    torch.cuda.synchronize()
+    for (_, _, _, _, out), ref in zip(keep, refs):
+        assert rel_l2(out.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
