@@ -16,6 +16,8 @@ struct tca_batch {
     tca::BatchUnitDev *d_units = nullptr;
     tca::ZeroDesc *d_zero = nullptr;
     int *d_counters = nullptr;
+    int *d_part_off = nullptr;   // first unit of each partition (kBatchParts + 1)
+    int nparts = 1;
 };
 
 namespace {
@@ -176,12 +178,47 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     for (auto &un : units)
         if (item_split[un.item]) un.atomic = 1;
 
-    // heaviest units first
+    // Partitions (one per XCD): units of one (item, group) -- its N-tiles,
+    // chunk pieces or SDDMM blocks, which read the same A tiles or G rows --
+    // go to one partition in bundles of up to kMaxBundle, so its L2 fetches
+    // them once per bundle (larger bundles balance the partitions worse).
+    // Bundles are assigned heaviest first to the least-loaded partition; inside
+    // a partition the units run heaviest first.
+    constexpr size_t kMaxBundle = 2;
+    const int grid = std::max(1, std::min((int)units.size(), workers));
+    const int nparts = std::min(tca::kBatchParts, grid);
     std::vector<int> order(units.size());
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<tca::BatchUnitDev> sorted(units.size());
-    for (size_t k = 0; k < order.size(); k++) sorted[k] = units[order[k]];
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return units[a].item != units[b].item ? units[a].item < units[b].item : units[a].g < units[b].g;
+    });
+    struct Bundle { size_t lo, hi; double cost; };
+    std::vector<Bundle> bundles;
+    for (size_t k = 0; k < order.size();) {
+        size_t e = k;
+        double c = 0.0;
+        while (e < order.size() && e - k < kMaxBundle && units[order[e]].item == units[order[k]].item &&
+               units[order[e]].g == units[order[k]].g)
+            c += cost[order[e++]];
+        bundles.push_back({k, e, c});
+        k = e;
+    }
+    std::stable_sort(bundles.begin(), bundles.end(), [](const Bundle &a, const Bundle &b) { return a.cost > b.cost; });
+    std::vector<std::vector<int>> part(nparts);
+    std::vector<double> load(nparts, 0.0);
+    for (const Bundle &bd : bundles) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += bd.cost;
+        for (size_t k = bd.lo; k < bd.hi; k++) part[x].push_back(order[k]);
+    }
+    std::vector<tca::BatchUnitDev> sorted;
+    std::vector<int> part_off(tca::kBatchParts + 1, 0);
+    for (int x = 0; x < nparts; x++) {
+        std::stable_sort(part[x].begin(), part[x].end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        part_off[x] = (int)sorted.size();
+        for (int k : part[x]) sorted.push_back(units[k]);
+    }
+    for (int x = nparts; x <= tca::kBatchParts; x++) part_off[x] = (int)sorted.size();
 
     tca_batch *b = new (std::nothrow) tca_batch();
     if (!b) return fail(TCA_ERR_ALLOC, "tca_batch_create: host allocation");
@@ -190,23 +227,26 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     b->n_zero = (int)zero.size();
     b->device = dev_id;
     b->dense_only = dense_only;
-    b->grid = std::max(1, std::min(b->n_units, workers));
+    b->grid = grid;
+    b->nparts = nparts;
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
     const size_t bz = sizeof(tca::ZeroDesc) * std::max<size_t>(zero.size(), 1);
     const size_t bc = (bi + bu + bz + 255) / 256 * 256;   // counters start on a 256-byte line
-    const size_t total = bc + tca::kBatchCounterBytes;
+    const size_t total = bc + tca::kBatchCounterBytes + 256;   // + partition offsets
     if (hipMalloc(&b->d_mem, total) != hipSuccess) { delete b; return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation"); }
     char *m = static_cast<char *>(b->d_mem);
     b->d_items = reinterpret_cast<tca::BatchItemDev *>(m);
     b->d_units = reinterpret_cast<tca::BatchUnitDev *>(m + bi);
     b->d_zero = reinterpret_cast<tca::ZeroDesc *>(m + bi + bu);
     b->d_counters = reinterpret_cast<int *>(m + bc);
+    b->d_part_off = reinterpret_cast<int *>(m + bc + tca::kBatchCounterBytes);
     if (hipMemcpy(b->d_items, dev.data(), bi, hipMemcpyHostToDevice) != hipSuccess ||
         (!sorted.empty() && hipMemcpy(b->d_units, sorted.data(), sizeof(tca::BatchUnitDev) * sorted.size(),
                                       hipMemcpyHostToDevice) != hipSuccess) ||
         (!zero.empty() && hipMemcpy(b->d_zero, zero.data(), sizeof(tca::ZeroDesc) * zero.size(),
                                     hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(b->d_part_off, part_off.data(), sizeof(int) * part_off.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_counters, 0, tca::kBatchCounterBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         tca_batch_destroy(b);
         return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
@@ -218,8 +258,9 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
 extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
 {
     if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
-    return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->grid,
-                             batch->d_zero, batch->n_zero, batch->dense_only, static_cast<hipStream_t>(stream));
+    return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->d_part_off,
+                             batch->nparts, batch->grid, batch->d_zero, batch->n_zero, batch->dense_only,
+                             static_cast<hipStream_t>(stream));
 }
 
 extern "C" tca_status tca_batch_destroy(tca_batch *batch)

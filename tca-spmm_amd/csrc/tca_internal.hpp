@@ -131,10 +131,10 @@ struct ZeroDesc {              // an output that split units accumulate into
     int32_t rows, N, ldc, pad;
 };
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
-// Work counters of a batch launch: the sorted unit list is dealt round-robin
-// to kBatchParts partitions (one per XCD under round-robin workgroup
-// dispatch), each with its own counter 4 KiB from the next (distinct memory
-// channels), then one count of finished workgroups.
+// Work counters of a batch launch: the unit list is cut into up to
+// kBatchParts partitions (one per XCD under round-robin workgroup dispatch),
+// each with its own counter 4 KiB from the next (distinct memory channels),
+// then one count of finished workgroups.
 constexpr int kBatchParts = 8;
 constexpr int kCounterStride = 1024;   // ints
 constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride * sizeof(int);
@@ -142,7 +142,8 @@ constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);
 // resident one-wave workgroups of batch_kernel per compute unit
 int batch_blocks_per_cu(bool dense_only);
-tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        const ZeroDesc *zero, int n_zero, bool dense_only, hipStream_t stream);
+tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters,
+                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, bool dense_only,
+                        hipStream_t stream);
 
 }  // namespace tca

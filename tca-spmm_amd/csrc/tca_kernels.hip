@@ -1378,8 +1378,8 @@ __global__ void warm_up_kernel() {}
 // Batched persistent kernel: every workgroup (kBatchWaves waves) takes work
 // units until the list is exhausted, so one launch covers the forward SpMM,
 // the transposed SpMM and the SDDMM of many layers.  Units are sorted heaviest
-// first on the host and dealt to kBatchParts partitions, each with its own
-// counter (see the loop below); a last counter counts finished workgroups, and
+// first on the host and cut into partitions, each with its own counter (see
+// the loop below); a last counter counts finished workgroups, and
 // the last workgroup resets them all, so the next launch (stream-ordered)
 // starts from zero without a memset.
 // Variants (few and small, so that the kernel's register budget -- the maximum
@@ -1465,18 +1465,22 @@ constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES
 template <bool DENSE_ONLY>
 __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                                  const BatchUnitDev *__restrict__ units, int n_units,
-                                                                 int *counters)
+                                                                 int *counters, const int *__restrict__ part_off,
+                                                                 int nparts)
 {
     __shared__ __attribute__((aligned(16))) uint8_t smem[kBatchLds];
     __shared__ int next_unit;
-    // Partition x of P holds units x, x + P, x + 2P, ... (heaviest first) and
-    // is served by the workgroups b with b % P == x.  A workgroup's first unit
-    // is static (its rank r in the partition); later ones come from the
-    // partition's counter, which hands out ranks gx + 0, 1, ....  One atomic
-    // address per partition keeps the claims from queueing on a single line.
-    const int P = min(kBatchParts, (int)gridDim.x);
+    // Partition x of P holds units [part_off[x], part_off[x+1]) (heaviest
+    // first) and is served by the workgroups b with b % P == x.  A workgroup's
+    // first unit is static (its rank r in the partition); later ones come from
+    // the partition's counter, which hands out ranks gx + 0, 1, ....  One
+    // atomic address per partition keeps the claims from queueing on a single
+    // line.  (P <= gridDim.x, so every partition has workgroups.)
+    (void)n_units;
+    const int P = nparts;
     const int x = (int)blockIdx.x % P, r = (int)blockIdx.x / P;
     const int gx = ((int)gridDim.x - x + P - 1) / P;
+    const int p_lo = part_off[x], p_n = part_off[x + 1] - p_lo;
     int *cnt = counters + x * kCounterStride;
     for (bool first = true;; first = false) {
         int k = r;
@@ -1486,9 +1490,8 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
             k = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
             __syncthreads();                                  // all have read next_unit
         }
-        const long uu = (long)k * P + x;
-        if (uu >= n_units) break;
-        const int u = (int)uu;
+        if (k >= p_n) break;
+        const int u = p_lo + k;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
@@ -1584,18 +1587,22 @@ int batch_blocks_per_cu(bool dense_only)
     return n;
 }
 
-tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters, int grid,
-                        const ZeroDesc *zero, int n_zero, bool dense_only, hipStream_t stream)
+tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters,
+                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, bool dense_only,
+                        hipStream_t stream)
 {
     if (n_zero > 0) {
         hipLaunchKernelGGL(zero_list_kernel, dim3(256, n_zero), dim3(256), 0, stream, zero);
         TCA_HIP_TRY(hipGetLastError());
     }
     if (n_units <= 0) return TCA_OK;
+    if (nparts < 1 || nparts > kBatchParts || nparts > grid) return fail(TCA_ERR_INVALID_ARG, "batch: partitions");
     if (dense_only)
-        hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
+        hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
+                           counters, part_off, nparts);
     else
-        hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units, counters);
+        hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
+                           counters, part_off, nparts);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }
