@@ -8,6 +8,8 @@ nearest even, the same bits the kernels would produce rounding on load), which
 halves the bytes both kernels gather.
 W's values live in *tiled order* (the plan's entry order) as the parameter,
 so no permutation runs per step; `SparseLinear.csr_values()` exports them.
+dense=True runs the forward and dX on panel-dense tiles built from the
+parameter in each call (one pack each; same bits, faster for large layers).
 """
 import numpy as np
 import torch
@@ -17,10 +19,13 @@ from .core import Plan, sddmm, spmm
 
 class SpmmFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, w_tiled, X, plan, plan_t, tile_n):
-        Y = spmm(plan, w_tiled, X, tile_n=tile_n)
+    def forward(ctx, w_tiled, X, plan, plan_t, tile_n, dense=False):
+        if dense:
+            Y = spmm(plan, plan.pack_dense(w_tiled), X, tile_n=tile_n, dense=True)
+        else:
+            Y = spmm(plan, w_tiled, X, tile_n=tile_n)
         ctx.save_for_backward(w_tiled, X)
-        ctx.plan, ctx.plan_t, ctx.tile_n = plan, plan_t, tile_n
+        ctx.plan, ctx.plan_t, ctx.tile_n, ctx.dense = plan, plan_t, tile_n, dense
         return Y
 
     @staticmethod
@@ -33,13 +38,16 @@ class SpmmFunction(torch.autograd.Function):
         if ctx.needs_input_grad[0]:
             dW = sddmm(ctx.plan, G, X).to(w_tiled.dtype)
         if ctx.needs_input_grad[1]:
-            dX = spmm(ctx.plan_t, w_tiled, G, tile_n=ctx.tile_n).to(X.dtype)
-        return dW, dX, None, None, None
+            if ctx.dense:
+                dX = spmm(ctx.plan_t, ctx.plan_t.pack_dense(w_tiled), G, tile_n=ctx.tile_n, dense=True).to(X.dtype)
+            else:
+                dX = spmm(ctx.plan_t, w_tiled, G, tile_n=ctx.tile_n).to(X.dtype)
+        return dW, dX, None, None, None, None
 
 
-def sparse_matmul(w_tiled, X, plan, plan_t, tile_n=0):
+def sparse_matmul(w_tiled, X, plan, plan_t, tile_n=0, dense=False):
     """Y (float32) = W X, differentiable in the tiled weight values and X."""
-    return SpmmFunction.apply(w_tiled, X, plan, plan_t, tile_n)
+    return SpmmFunction.apply(w_tiled, X, plan, plan_t, tile_n, dense)
 
 
 class SparseLinear(torch.nn.Module):
@@ -50,7 +58,7 @@ class SparseLinear(torch.nn.Module):
     """
 
     def __init__(self, n_out, n_in, row_offset, col_idx, values_csr, rows_per_wave=16, rows_per_wave_t=None,
-                 tile_n=0, device="cuda"):
+                 tile_n=0, device="cuda", dense=False):
         super().__init__()
         self.plan = Plan(n_out, n_in, row_offset, col_idx, rows_per_wave=rows_per_wave)
         self.plan_t = self.plan.transpose(rows_per_wave_t or rows_per_wave)
@@ -58,9 +66,12 @@ class SparseLinear(torch.nn.Module):
             np.ascontiguousarray(values_csr).view(np.int16)).view(torch.float16)
         self.weight = torch.nn.Parameter(self.plan.pack(v.to(device)))
         self.tile_n = tile_n
+        if dense and max(self.plan.rows_per_wave, self.plan_t.rows_per_wave) > 64:
+            raise ValueError("dense tiles need plans of 16, 32 or 64 rows per group")
+        self.dense = dense
 
     def forward(self, X):
-        return sparse_matmul(self.weight, X, self.plan, self.plan_t, self.tile_n)
+        return sparse_matmul(self.weight, X, self.plan, self.plan_t, self.tile_n, self.dense)
 
     def csr_values(self, tiled=None):
         return self.plan.unpack((self.weight if tiled is None else tiled).detach())

@@ -93,3 +93,22 @@ def test_gpu_gradients_against_dense_fp64(cuda, problem):
     dX32 = T.spmm(layer.plan_t, layer.weight.detach(), G).cpu().numpy()
     ref_dx = O.spmm_f64(p["K"], p["N"], p["rpT"], p["cT"], p["w64"][p["perm"]], g16)
     assert rel_l2(dX32, ref_dx) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_dense_tiles_match_tiled(cuda, problem):
+    """SparseLinear(dense=True) -- forward and dX on panel-dense tiles packed
+    from the parameter in each call -- gives the tiled-values layer's Y, dW and
+    dX bit for bit."""
+    p = problem
+    W = p["W"]
+    out = []
+    for dense in (False, True):
+        layer = T.SparseLinear(p["M"], p["K"], W.row_offset, W.col_idx, W.values, rows_per_wave=32,
+                               rows_per_wave_t=16, tile_n=64, dense=dense)
+        X = dev_f16(p["X"]).requires_grad_(True)
+        Y = layer(X)
+        (Y * dev(p["G"])).sum().backward()
+        out.append((Y.detach(), layer.weight.grad, X.grad))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
