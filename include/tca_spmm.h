@@ -214,9 +214,10 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 /* Batched persistent launch over many layers (one kernel, capturable).      */
 /* SPMM items: F16 values; forward with an F16 B on 16-row plans (tile_n     */
 /* 16/32/64) or 32-row plans (tile_n 32); transposed with an F32 or F16 B on  */
-/* 16-row plans (tile_n 16/32/64); panel-dense F16 A (a_dense) with an F16  */
-/* or F32 B on 16-row plans, forward or transposed (tile_n 16/32/64).  SDDMM */
-/* items: F16 X, F32 or F16 G.  Other combinations: TCA_ERR_UNSUPPORTED.     */
+/* 16-row plans (tile_n 16/32/64); panel-dense F16 A (a_dense), forward or  */
+/* transposed, with an F16 B on 16-row (tile_n 16/32/64), 32-row (32/64) or  */
+/* 64-row (32) plans, or an F32 B on 16-row (16/32/64) or 32-row (32) plans. */
+/* SDDMM items: F16 X, F32 or F16 G.  Other combinations: UNSUPPORTED.       */
 /* ------------------------------------------------------------------------ */
 typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
 typedef struct tca_batch_item {

@@ -285,16 +285,25 @@ struct SpmmParams {
 //   xcd_share == 0: n_tiles % 8 == 0, XCD x runs tiles x, x+8, ... (all units);
 //   xcd_share == k (1..8, k * n_tiles == 8): k XCDs share a tile, taking its
 //     units round-robin so that heavy and light units stay spread;
-//   xcd_share < 0: plain order (tile = L / n_units).
+//   xcd_share == -2: units are dealt to XCDs instead (XCD x runs units x,
+//     x+8, ..., each with all its N-tiles), for A operands that outweigh B:
+//     a unit's A is then fetched by one L2, not by one per N-tile;
+//   xcd_share == -1: plain order (tile = L / n_units).
 // Returns false for the padding workgroups of a grid rounded up to 8.
 __device__ __forceinline__ bool xcd_work(const SpmmParams &p, int L, int &unit, int &tile)
 {
-    if (p.xcd_share < 0) {
+    if (p.xcd_share == -1) {
         tile = L / p.n_units;
         unit = L - tile * p.n_units;
         return true;
     }
     const int x = L & 7, l = L >> 3;
+    if (p.xcd_share == -2) {
+        const int r = l / p.n_tiles;
+        tile = l - r * p.n_tiles;
+        unit = x + 8 * r;
+        return unit < p.n_units;
+    }
     if (p.xcd_share == 0) {
         const int r = l / p.n_units;
         unit = l - r * p.n_units;
@@ -1759,7 +1768,16 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     if (n_units <= 0) return TCA_OK;
     prm.n_units = n_units;
     long grid;
-    if (prm.n_tiles % 8 == 0) {
+    // A read once per N-tile pass (tile-major) against B slabs read by every
+    // XCD (unit-major): deal whichever is heavier once
+    const int es_a = a_dtype == TCA_E4M3 ? 1 : 2;
+    const double a_bytes = dense ? (double)dense_bytes(t, a_dtype) : (double)t.nnz * (es_a + 2 + (p->transposed ? 4 : 0));
+    const double b_bytes = (double)t.n_cols * N * (b_dtype == TCA_F32 ? 4 : b_dtype == TCA_E4M3 ? 1 : 2);
+    const int tiles_per_xcd_pass = std::min(prm.n_tiles, 8);
+    if (t.rpw < 128 && prm.n_tiles > 1 && a_bytes * (tiles_per_xcd_pass - 1) > b_bytes * 7) {
+        prm.xcd_share = -2;
+        grid = 8L * ((n_units + 7) / 8) * prm.n_tiles;
+    } else if (prm.n_tiles % 8 == 0) {
         prm.xcd_share = 0;
         grid = (long)n_units * prm.n_tiles;
     } else if (8 % prm.n_tiles == 0) {
