@@ -191,3 +191,47 @@ def test_dense_batch_wider_plans(cuda, rpw, tile, g_f32):
     torch.cuda.synchronize()
     for (_, _, _, _, out), ref in zip(keep, refs):
         assert rel_l2(out.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+
+
+def test_dense_empty_plan_and_empty_groups(cuda):
+    """nnz = 0: no tiles, C all zeros; a plan whose only entries sit in one
+    group: the dense launch equals the tiled one and zero rows stay zero."""
+    M, K, N = 70, 30, 48
+    B = dev_f16(gen.f32_to_f16_bits(gen.normal_f32(3, 1, (K, N))))
+    empty = T.Plan(M, K, np.zeros(M + 1, np.int32), np.zeros(0, np.int32), 32)
+    assert empty.dense_elems == 0
+    tiles = empty.pack_dense(torch.zeros(1, dtype=torch.float16, device=cuda)[:0])
+    C = T.spmm(empty, tiles, B, dense=True)
+    torch.cuda.synchronize()
+    assert tiles.numel() == 0 and torch.count_nonzero(C).item() == 0
+    rp = np.zeros(M + 1, np.int32)
+    rp[65:] = 2
+    plan = T.Plan(M, K, rp, np.array([3, 29], np.int32), 64)
+    tiled = plan.pack(dev_f16(np.array([0x3C00, 0xC000], np.uint16)))
+    for t in TILES:
+        got = T.spmm(plan, plan.pack_dense(tiled), B, tile_n=t, dense=True)
+        assert torch.equal(got, T.spmm(plan, tiled, B, tile_n=t))
+        assert torch.count_nonzero(torch.cat([got[:64], got[65:]])).item() == 0
+
+
+def test_dense_transposed_e4m3_and_bf16(cuda):
+    """Transposed plans with E4M3 (scaled) and BF16 values: dense tiles built
+    through the entry map give the indirect launch's bits."""
+    case = workloads.sparse_linear(M=1024, K=512, N=128)
+    W, G = case.A, case.extra["G"]
+    fwd = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+    q, s = T.quantize_rows_e4m3(dev(W.row_offset), dev_f16(W.values))
+    gq, gs = T.quantize_cols_e4m3(dev_f16(G.astype(np.float16).view(np.uint16)))
+    bf = dev_bf16(gen.f32_to_bf16_bits(W.values.view(np.float16).astype(np.float32)))
+    gbf = dev_bf16(gen.f32_to_bf16_bits(G))
+    for rpw in RPWS:
+        tr = fwd.transpose(rpw)
+        for vals, Gd in ((fwd.pack(q), gq), (fwd.pack(bf), gbf)):
+            dense = tr.pack_dense(vals)
+            assert np.array_equal(dense.cpu().view(torch.uint8).numpy() if dense.dtype == torch.uint8 else
+                                  dense.cpu().view(torch.int16).numpy(),
+                                  host_dense(tr, vals.cpu().view(torch.uint8).numpy() if vals.dtype == torch.uint8 else
+                                             vals.cpu().view(torch.int16).numpy(), tr.export()["entry_src"]))
+            for t in (32, 64):
+                sc = dict(col_scale=gs) if vals.dtype == torch.uint8 else {}
+                assert torch.equal(T.spmm(tr, dense, Gd, tile_n=t, dense=True, **sc), T.spmm(tr, vals, Gd, tile_n=t, **sc))
