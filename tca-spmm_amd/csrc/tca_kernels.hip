@@ -1141,6 +1141,8 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     // sampling indices of this wave's panel in each chunk, issued now so that
     // they arrive during the contraction (out-of-range entries read as zero
     // and are skipped when sampling)
+    // (all bounds first: interleaving them with the entry loads would make
+    // each chunk's bounds wait for the previous chunk's entries)
     int e0[CB], e1[CB];
     uint32_t slot[CB][kShEP], dsti[CB][kShEP];
 #pragma unroll
@@ -1148,6 +1150,9 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
         const int c = c0 + min(b, nc - 1);
         e0[b] = gbl(p.seg_ptr)[c * plan_ppg + w];
         e1[b] = gbl(p.seg_ptr)[c * plan_ppg + w + 1];
+    }
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
 #pragma unroll
         for (int k = 0; k < kShEP; k++) {
             const uint32_t e = (uint32_t)(e0[b] + k * 64 + lane);
@@ -1177,38 +1182,50 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
         for (int j = 0; j < L::XP; j++)
             if (xval[j]) *reinterpret_cast<Oct *>(buf + xdst[j]) = xr[slot_][j];
     };
-    // prologue: X of step 0 in LDS; X of steps 1, 2 and G of steps 0, 1 in registers
+    // prologue: X of step 0 in LDS; G of steps 0, 1 and X of steps 1, 2 in
+    // registers, issued in the loop's own order (G(s), X(s+1), G(s+1), X(s+2)
+    // in flight at the top of step s) so that the compiler's waits in the
+    // loop, derived from its entry and back-edge states together, stay those
+    // of the steady state
+    // (scheduling barriers keep the compiler from reordering these loads)
     load_x(0, 0);
-    gr[0] = GB::load(r_g, gsrc, 0);
-    gr[1] = GB::load(r_g, gsrc, (uint32_t)(32 * min(1, steps - 1)) * GB::BES);
     put_x(0, 0);
+    gr[0] = GB::load(r_g, gsrc, 0);
+    __builtin_amdgcn_sched_barrier(0);
     load_x(1, min(1, steps - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    gr[1] = GB::load(r_g, gsrc, (uint32_t)(32 * min(1, steps - 1)) * GB::BES);
+    __builtin_amdgcn_sched_barrier(0);
     load_x(0, min(2, steps - 1));
     __syncthreads();
-    for (int s0 = 0; s0 < steps; s0 += 2) {
+    auto step = [&](int k, int s) {   // k = s & 1, a compile-time register slot
+        const uint8_t *buf = smem + (s & 1) * L::XT;
+        const Oct ga = GR::cvt(gr[k]);
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int s = s0 + k;
-            if (s < steps) {
-                const uint8_t *buf = smem + (s & 1) * L::XT;
-                const Oct ga = GR::cvt(gr[k]);
+        for (int b = 0; b < CB; b++) {
+            if (b < nc) {
 #pragma unroll
-                for (int b = 0; b < CB; b++) {
-                    if (b < nc) {
-#pragma unroll
-                        for (int h = 0; h < 2; h++) {
-                            const Oct xb = *reinterpret_cast<const Oct *>(buf + xread + (32 * b + 16 * h) * kShPitch * 2);
-                            acc[b][h] = Elt<AT>::mma(ga, xb, acc[b][h]);
-                        }
-                    }
+                for (int h = 0; h < 2; h++) {
+                    const Oct xb = *reinterpret_cast<const Oct *>(buf + xread + (32 * b + 16 * h) * kShPitch * 2);
+                    acc[b][h] = Elt<AT>::mma(ga, xb, acc[b][h]);
                 }
-                gr[k] = GB::load(r_g, gsrc, (uint32_t)(32 * min(s + 2, steps - 1)) * GB::BES);
-                put_x(1 - k, s + 1);                      // X of step s+1, loaded two steps ago
-                load_x(1 - k, min(s + 3, steps - 1));     // (clamped re-reads past the end)
-                __syncthreads();
             }
         }
+        gr[k] = GB::load(r_g, gsrc, (uint32_t)(32 * min(s + 2, steps - 1)) * GB::BES);
+        put_x(1 - k, s + 1);                      // X of step s+1, loaded two steps ago
+        load_x(1 - k, min(s + 3, steps - 1));     // (clamped re-reads past the end)
+        __syncthreads();
+    };
+    // Pairs of steps with no guard inside (a path that skipped the second
+    // step's loads would reach the loop head with fewer loads in flight, and
+    // the compiler would size every iteration's waits for that path), then
+    // the odd last step.
+    int s0 = 0;
+    for (; s0 + 1 < steps; s0 += 2) {
+        step(0, s0);
+        step(1, s0 + 1);
     }
+    if (s0 < steps) step(0, s0);
     // sample this wave's panel: stage each chunk's 16x32 tile in the wave's area
     float *tile = reinterpret_cast<float *>(smem + 2 * L::XT + w * L::STAGE);
 #pragma unroll
