@@ -184,9 +184,10 @@ tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_til
                     tca_dtype b_dtype, const void *d_B, int ldb, int N,
                     float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
                     int tile_n, void *stream);
-/* tca_spmm with A as panel-dense tiles (tca_pack_dense) of a plan of 16, 32
- * or 64 rows per group, forward or transposed; same arguments and results
- * (bit for bit: the MFMAs see the same operands in the same order). */
+/* tca_spmm with A as panel-dense tiles (tca_pack_dense), forward or
+ * transposed plan (128 / 256 rows per group: F16 / BF16 A); same arguments
+ * and results (bit for bit: the MFMAs see the same operands in the same
+ * order). */
 tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_dense,
                           tca_dtype b_dtype, const void *d_B, int ldb, int N,
                           float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,

@@ -770,9 +770,10 @@ struct WideLds {
     static constexpr int PPT = (PIECES + WM * 64 - 1) / (WM * 64);  // pieces per thread
 };
 
-template <int PPW, int NT, int WM, tca_dtype AT, tca_dtype BT, bool INDIRECT>
+template <int PPW, int NT, int WM, tca_dtype AT, tca_dtype BT, bool INDIRECT, bool DENSE = false>
 __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 {
+    static_assert(!(DENSE && INDIRECT), "panel-dense tiles are packed in the plan's own order");
     using L = WideLds<NT, WM, PPW>;
     using RL = RowLoad<AT, BT>;
     using RB = RowBuf<AT, BT>;
@@ -799,6 +800,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     const Rsrc r_lds = make_rsrc(p.entry_lds, nnz * 2u);
     const Rsrc r_val = make_rsrc(p.a, nnz * 2u);
     const Rsrc r_src = make_rsrc(p.entry_src, nnz * 4u);
+    const Rsrc r_dense = make_rsrc(p.a, (uint32_t)p.n_chunks * (uint32_t)(PPG * 64 * 16));
     const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
 
     // this thread's B pieces: row k, 8-column segment
@@ -816,6 +818,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     struct Index {
         int col[L::PPT];
         int segb;
+        int c;                         // the chunk (DENSE: its tiles' index)
     };
     struct Stage {
         typename RB::Raw rows[L::PPT];  // absent rows / tail columns read as zeros
@@ -824,6 +827,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         uint32_t val[PPW][EP];         // value bits, or entry_src when INDIRECT
         uint32_t vv[PPW][EP];          // INDIRECT: value bits, loaded one chunk ahead
         int e0[PPW], e1[PPW];
+        Oct aop[PPW];                  // DENSE: this lane's A operand octet of each of its panels
     };
 
     f32x4 acc[PPW][NSUB];
@@ -831,13 +835,16 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     for (int i = 0; i < PPW; i++)
 #pragma unroll
         for (int s = 0; s < NSUB; s++) acc[i][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (!DENSE) {
 #pragma unroll
-    for (int i = 0; i < PPW; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+        for (int i = 0; i < PPW; i++) reinterpret_cast<Oct *>(lds_a)[i * 64 + lane] = zero_oct<Oct>();
+    }
 
     auto load_index = [&](int c, Index &ix) {
 #pragma unroll
         for (int j = 0; j < L::PPT; j++) ix.col[j] = p.chunk_cols[(long)c * 32 + pk[j]];
-        ix.segb = gbl(p.seg_ptr)[(long)c * PPG + w * PPW + seg_lane];
+        ix.c = c;
+        if constexpr (!DENSE) ix.segb = gbl(p.seg_ptr)[(long)c * PPG + w * PPW + seg_lane];
     };
     auto load_data = [&](const Index &ix, Stage &st) {
         st.eok = 0;
@@ -846,6 +853,12 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
             const int ncol = n0 + 8 * pseg[j];
             const bool ok = pvalid[j] && ix.col[j] >= 0 && ncol < p.N;
             st.rows[j] = RB::load(r_b, ok ? (uint32_t)ix.col[j] * ldb_bytes + (uint32_t)ncol * RB::BES : kOOB);
+        }
+        if constexpr (DENSE) {
+#pragma unroll
+            for (int i = 0; i < PPW; i++)
+                st.aop[i] = buf_b128(r_dense, ((uint32_t)(ix.c * PPG + w * PPW + i) * 64u + (uint32_t)lane) * 16u);
+            return;
         }
 #pragma unroll
         for (int i = 0; i < PPW; i++) {
@@ -882,7 +895,22 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
                 *reinterpret_cast<Oct *>(buf + (pk[j] * L::LDB + 8 * pseg[j]) * 2) = v;
             }
     };
+    auto compute_dense = [&](const Stage &st, const uint8_t *buf) {
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            const int off = (brow * L::LDB + 16 * s + bcol) * 2;
+            const uint2 lo = lds_read_tr16(buf, off);
+            const uint2 hi = lds_read_tr16(buf, off + 4 * L::LDB * 2);
+            const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
+#pragma unroll
+            for (int i = 0; i < PPW; i++) acc[i][s] = Elt<AT>::mma(st.aop[i], b, acc[i][s]);
+        }
+    };
     auto compute = [&](const Stage &st, const uint8_t *buf) {
+        if constexpr (DENSE) {
+            compute_dense(st, buf);
+            return;
+        }
         E v[PPW][EP];
 #pragma unroll
         for (int i = 0; i < PPW; i++)
@@ -1006,22 +1034,22 @@ SpmmFn pick_rpw(int rpw, int nt)
     return nullptr;
 }
 
-template <int PPW, int WM, tca_dtype AT, tca_dtype BT, bool IND>
+template <int PPW, int WM, tca_dtype AT, tca_dtype BT, bool IND, bool DN = false>
 SpmmFn pick_wide_nt(int nt)
 {
     switch (nt) {
-    case 64: return spmm_wide_kernel<PPW, 64, WM, AT, BT, IND>;
-    case 128: return spmm_wide_kernel<PPW, 128, WM, AT, BT, IND>;
+    case 64: return spmm_wide_kernel<PPW, 64, WM, AT, BT, IND, DN>;
+    case 128: return spmm_wide_kernel<PPW, 128, WM, AT, BT, IND, DN>;
     }
     return nullptr;
 }
 
-template <tca_dtype AT, tca_dtype BT, bool IND>
+template <tca_dtype AT, tca_dtype BT, bool IND, bool DN = false>
 SpmmFn pick_wide_rpw(int rpw, int nt)
 {
     switch (rpw) {
-    case 128: return pick_wide_nt<2, 4, AT, BT, IND>(nt);
-    case 256: return pick_wide_nt<2, 8, AT, BT, IND>(nt);
+    case 128: return pick_wide_nt<2, 4, AT, BT, IND, DN>(nt);
+    case 256: return pick_wide_nt<2, 8, AT, BT, IND, DN>(nt);
     }
     return nullptr;
 }
@@ -1044,9 +1072,16 @@ SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
     return nullptr;
 }
 
-// Panel-dense A (plans of 16 / 32 / 64 rows per group, forward or transposed).
+// Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A).
 SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt)
 {
+    if (rpw >= 128) {
+        if (at == TCA_F16 && bt == TCA_F16) return pick_wide_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_BF16) return pick_wide_rpw<TCA_BF16, TCA_BF16, false, true>(rpw, nt);
+        if (at == TCA_F16 && bt == TCA_F32) return pick_wide_rpw<TCA_F16, TCA_F32, false, true>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_F32) return pick_wide_rpw<TCA_BF16, TCA_F32, false, true>(rpw, nt);
+        return nullptr;
+    }
     if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
     if (at == TCA_BF16 && bt == TCA_BF16) return pick_rpw<TCA_BF16, TCA_BF16, false, true>(rpw, nt);
     if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_rpw<TCA_E4M3, TCA_E4M3, false, true>(rpw, nt);
@@ -1717,7 +1752,6 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
 {
     const PlanTables &t = p->host;
     if (tile_n == 0) tile_n = default_tile_n(p, N);
-    if (dense && t.rpw >= 128) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_dense: plans of 16, 32 or 64 rows per group");
     if (dense && dense_bytes(t, a_dtype) >= (1L << 31))
         return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_dense: the dense tiles reach 2 GiB (32-bit offsets)");
     SpmmFn fn = dense           ? pick_spmm_dense(a_dtype, b_dtype, t.rpw, tile_n)

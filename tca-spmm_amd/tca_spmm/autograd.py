@@ -66,8 +66,6 @@ class SparseLinear(torch.nn.Module):
             np.ascontiguousarray(values_csr).view(np.int16)).view(torch.float16)
         self.weight = torch.nn.Parameter(self.plan.pack(v.to(device)))
         self.tile_n = tile_n
-        if dense and max(self.plan.rows_per_wave, self.plan_t.rows_per_wave) > 64:
-            raise ValueError("dense tiles need plans of 16, 32 or 64 rows per group")
         self.dense = dense
 
     def forward(self, X):

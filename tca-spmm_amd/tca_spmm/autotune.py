@@ -9,7 +9,7 @@ fastest first.  Results recorded with `record()` go to tuning_table.json next
 to this file; `choice_for()` looks up the nearest recorded shape (same M, K,
 N class, closest density) and falls back to a density rule when none exists.
 Entries are kept per A layout: tiled values, or panel-dense tiles
-(`layout="dense"`, plans of 16/32/64 rows only).
+(`layout="dense"`).
 """
 import json
 import math
@@ -78,16 +78,14 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False, dense=False):
             best, best_dist = e, dist
     if best is not None and best_dist < 0.7:
         return best["rows_per_wave"], best["tile_n"]
-    rpw, tile = heuristic(M, K, nnz, N)
-    return (min(rpw, 64), tile) if dense else (rpw, tile)
+    return heuristic(M, K, nnz, N)
 
 
 def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", transposed=False, dense=False):
     """Time each (rows_per_wave, tile_n) on the device.  A: workloads.SparseMatrix
     (fp16 values), B_dev: K x N device tensor (for transposed=True: the
     M x N float32 gradient, and the product is A^T B).  dense=True times A as
-    panel-dense tiles (16/32/64-row candidates).  Returns a list of dicts
-    sorted by time (us per launch)."""
+    panel-dense tiles.  Returns a list of dicts sorted by time (us per launch)."""
     import numpy as np
     import torch
 
@@ -97,8 +95,7 @@ def tune(A, B_dev, values_dev=None, candidates=None, iters=30, device="cuda", tr
         values_dev = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(device)
     N = B_dev.shape[1]
     cands = candidates or ALL_CANDIDATES
-    if dense:
-        cands = [c for c in cands if c[0] in RPW_CANDIDATES]
+
     out = torch.empty((A.n_cols if transposed else A.n_rows, N), dtype=torch.float32, device=device)
     stream = torch.cuda.current_stream()
     res = []

@@ -162,9 +162,39 @@ def test_dense_rejections(cuda):
     with pytest.raises(ValueError):
         T.spmm(plan, tiled, dev_f16(case.B), dense=True)            # tiled values, not tiles
     wide = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, 128)
-    with pytest.raises(T.TcaError) as e:
-        T.spmm(wide, wide.pack_dense(wide.pack(dev_f16(A.values))), dev_f16(case.B), dense=True)
+    q, _ = T.quantize_rows_e4m3(dev(A.row_offset), dev_f16(A.values))
+    qb, _ = T.quantize_cols_e4m3(dev_f16(case.B))
+    with pytest.raises(T.TcaError) as e:                             # wide groups: F16 / BF16 only
+        T.spmm(wide, wide.pack_dense(wide.pack(q)), qb, dense=True)
     assert e.value.name == "TCA_ERR_UNSUPPORTED"
+
+
+@pytest.mark.parametrize("rpw", [128, 256])
+def test_dense_wide_plans(cuda, rpw):
+    """Wide groups (one workgroup of rows/32 waves sharing each B chunk) on
+    panel-dense tiles: forward (F16 and BF16) and transposed (F32 gradient),
+    tile_n 64 / 128, against the tiled-values launch.  These launches split
+    the few long groups over workgroups that add atomically, so the two agree
+    to FP32 summation order, not bit for bit."""
+    case = workloads.sparse_linear(M=2048, K=1024, N=256)
+    W, X, G = case.A, case.B, case.extra["G"]
+    plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+    bf = gen.f32_to_bf16_bits(W.values.view(np.float16).astype(np.float32))
+    Xbf = dev_bf16(gen.f32_to_bf16_bits(X.view(np.float16).astype(np.float32)))
+    for vals, B in ((dev_f16(W.values), dev_f16(X)), (dev_bf16(bf), Xbf)):
+        tiled = plan.pack(vals)
+        dense = plan.pack_dense(tiled)
+        assert np.array_equal(dense.cpu().view(torch.int16).numpy(),
+                              host_dense(plan, tiled.cpu().view(torch.int16).numpy()))
+        for t in (64, 128):
+            got = T.spmm(plan, dense, B, tile_n=t, dense=True).cpu().numpy()
+            assert rel_l2(got, T.spmm(plan, tiled, B, tile_n=t).cpu().numpy()) <= 1e-6
+    fwd16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+    w16 = fwd16.pack(dev_f16(W.values))
+    tr = fwd16.transpose(rpw)
+    for t in (64, 128):
+        got = T.spmm(tr, tr.pack_dense(w16), dev(G), tile_n=t, dense=True).cpu().numpy()
+        assert rel_l2(got, T.spmm(tr, w16, dev(G), tile_n=t).cpu().numpy()) <= 1e-6
 
 
 @pytest.mark.parametrize("rpw,tile,g_f32", [(32, 32, False), (32, 64, False), (64, 32, False), (32, 32, True)])
