@@ -415,6 +415,14 @@ def main():
     del flush
     extra["per_matrix"] = per
 
+    # achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
+    src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    copy_ms = time_eager(torch, lambda: dst.copy_(src), 10, backlog=True)
+    copy_gbs = 2 * src.numel() / (copy_ms * 1e-3) / 1e9      # bytes read + written
+    del src, dst
+
     # library baselines on the same inputs
     if not args.no_libs:
         lib_ms = {"rocsparse": 0.0, "hipblaslt": 0.0}
@@ -495,7 +503,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch_plain_gpu_suite_traffic"),
                      "kernel": "batch_kernel (the step's one launch: ten products' algorithmic bytes / the "
                                "launch's duration, event pair around K back-to-back launches)",
-                     "alg_bytes_per_step": total_bytes, "launch_us_mean": launch_us},
+                     "alg_bytes_per_step": total_bytes, "launch_us_mean": launch_us,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "extra": extra,
     }
