@@ -415,6 +415,30 @@ def main():
     del flush
     extra["per_matrix"] = per
 
+    # Blocked-ELL (the reference's block-SpMM baseline, a gfx950 kernel here) on
+    # the same ten products, blocks of 16 and 32: the format pads every block
+    # row to the longest one and every kept block to b x b, so it moves and
+    # multiplies `fill` times the nonzeros
+    bell = {"rows": [], "ms_per_step": {}}
+    for b in (16, 32):
+        tot = 0.0
+        for i, e in enumerate(entries):
+            A = e["case"].A
+            M_ = T.BlockedEll.from_csr(A.n_rows, A.n_cols, A.row_offset, A.col_idx, A.values, b)
+            Cb = torch.empty_like(e["C"])
+            us = time_eager(torch, lambda: T.bell_spmm(M_, e["B"], out=Cb), 50, backlog=True) * 1e3
+            launch(e, "d")
+            torch.cuda.synchronize()
+            rel = float(torch.linalg.norm(Cb - e["C"]) / torch.linalg.norm(e["C"]))
+            if b == 16:
+                bell["rows"].append({"name": A.name})
+            bell["rows"][i][f"b{b}_us"] = us
+            bell["rows"][i][f"b{b}_fill"] = M_.values.numel() / max(A.nnz, 1)
+            bell["rows"][i][f"b{b}_rel_vs_spmm"] = rel
+            tot += us
+        bell["ms_per_step"][f"b{b}"] = tot * 1e-3
+    extra["blocked_ell"] = bell
+
     # achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
     src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
     dst = torch.empty_like(src)

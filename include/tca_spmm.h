@@ -193,6 +193,36 @@ tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d
                           float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
                           int tile_n, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Blocked-ELL.  Replaces perform_cuSPARSE_BlockSpMM (src/spmm_cusparse.cu:   */
+/* 181-445, decl src/spmm_cusparse.cuh:14): its host conversion (:254-331)   */
+/* and the cusparseCreateBlockedEll + cusparseSpMM pair (:337-343, :394-398). */
+/* Format: ceil(M/block) block rows of slots = ell_cols/block block slots;    */
+/* col_ind[br*slots + j] = block column of slot j, or -1 (empty: its values  */
+/* are not read); values (ceil(M/block)*block) x ell_cols row-major, row r,   */
+/* slot j, column c of the block at values[r*ell_cols + j*block + c].  Values */
+/* at columns >= n_cols of the last block column must be finite (zeros).      */
+/* ------------------------------------------------------------------------ */
+/* Host conversion from CSR (values 16-bit, F16 or BF16 bits; NULL = pattern
+ * only, 1.0 at every entry like the .smtx reader).  A block is kept when it
+ * holds a CSR entry; kept block columns ascending, -1 after the last.  Call
+ * with col_ind / ell_values NULL to get ell_cols, then with buffers of
+ * ceil(M/block)*(ell_cols/block) and ceil(M/block)*block*ell_cols elements.
+ * block in {16, 32, 64, 128, 256}. */
+tca_status tca_bell_from_csr(int n_rows, int n_cols, const int32_t *row_offset, const int32_t *col_idx,
+                             const uint16_t *values, int block, int *ell_cols, int32_t *col_ind,
+                             uint16_t *ell_values);
+/* C[M x N] (f32, every row written) = A (Blocked-ELL, device arrays) * B (K x N
+ * row-major, F16/BF16 like the values, or F32 rounded on load).  N % 8 == 0;
+ * values, B, C 16-byte aligned, ldb % 8 == 0, ldc % 4 == 0.  tile_n (0 = auto,
+ * 16/32/64) and rows_per_wave (0 = auto, 16/32/64, at most block) fix the
+ * kernel's work unit. */
+tca_status tca_bell_spmm(int n_rows, int n_cols, int block, int ell_cols, const int32_t *d_col_ind,
+                         tca_dtype a_dtype, const void *d_values, tca_dtype b_dtype, const void *d_B, int ldb,
+                         int N, float *d_C, int ldc, int tile_n, int rows_per_wave, void *stream);
+/* The (rows_per_wave, tile_n) tca_bell_spmm picks for a shape when given 0s. */
+tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile);
+
 /* SDDMM on the plan's pattern: out_tiled[e] = sum_n G[row(e), n] * X[col(e), n]
  * (the dW of Y = W X with dY = G).  G: M x N row-major (ldg), X: K x N (ldx),
  * dtypes F16/BF16, or F32 (G) rounded to X's type on load.  N % 32 == 0.

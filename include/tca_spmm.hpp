@@ -37,6 +37,10 @@ void our_spmm_balanced(CSR *A_csr, half *B, float *C, const int M, const int N, 
 /* src/spmm_cusparse.cuh:15 and src/gemm_cublas.cuh:12, on rocSPARSE / hipBLASLt */
 void perform_rocSPARSE_SpMM(half *A, half *B, float *C, int M, int N, int K, int num_iter, double *elapsed);
 void perform_hipBLASLt_GemmEx(half *A, half *B, float *C, int M, int N, int K, int n_iter, double *elapsed);
+/* src/spmm_cusparse.cuh:14 (perform_cuSPARSE_BlockSpMM): dense A -> Blocked-ELL
+ * with blocks of ell_block_size (16/32/64/128/256), then the gfx950 Blocked-ELL
+ * kernel (tca_bell_spmm); C (M x N) is overwritten. */
+void perform_BlockSpMM(half *A, half *B, float *C, int M, int N, int K, int num_iter, int ell_block_size, double *elapsed);
 /* src/miscalleneous.cu:158-163 (warm_up_gpu_kernel<<<1,32>>>) */
 void warm_up_gpu();
 

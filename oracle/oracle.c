@@ -520,3 +520,58 @@ void orc_transpose_csr(int M, int K, const int *rowptr, const int *colidx,
         }
     free(fill);
 }
+
+/* src/spmm_cusparse.cu:254-331 (perform_cuSPARSE_BlockSpMM's host conversion) */
+int orc_dense_to_bell(const uint16_t *dense, int M, int K, int b, int32_t *col_ind, uint16_t *values)
+{
+    const int BR = (M + b - 1) / b, BC = (K + b - 1) / b;
+    unsigned char *kept = (unsigned char *)calloc((size_t)(BR > 0 ? BR : 1) * (size_t)(BC > 0 ? BC : 1), 1);
+    for (int r = 0; r < M; r++)
+        for (int c = 0; c < K; c++)
+            if (orc_f16_to_f32(dense[(size_t)r * K + c]) != 0.0f) kept[(size_t)(r / b) * BC + c / b] = 1;
+    int slots = 0;
+    for (int br = 0; br < BR; br++) {
+        int n = 0;
+        for (int bc = 0; bc < BC; bc++) n += kept[(size_t)br * BC + bc];
+        if (n > slots) slots = n;
+    }
+    const int ell_cols = slots * b;
+    if (col_ind && values) {
+        for (int br = 0; br < BR; br++) {
+            int j = 0;
+            for (int bc = 0; bc < BC; bc++)
+                if (kept[(size_t)br * BC + bc]) col_ind[(size_t)br * slots + j++] = bc;
+            for (; j < slots; j++) col_ind[(size_t)br * slots + j] = -1;
+        }
+        for (int r = 0; r < BR * b; r++)
+            for (int e = 0; e < ell_cols; e++) {
+                const int bc = col_ind[(size_t)(r / b) * slots + e / b];
+                const int c = bc * b + e % b;
+                values[(size_t)r * ell_cols + e] = (bc >= 0 && r < M && c < K) ? dense[(size_t)r * K + c] : 0;
+            }
+    }
+    free(kept);
+    return ell_cols;
+}
+
+/* cusparseSpMM on the Blocked-ELL descriptor, src/spmm_cusparse.cu:394-398 */
+void orc_bell_spmm_f32(int M, int K, int N, int b, int ell_cols, const int32_t *col_ind,
+                       const uint16_t *values, const uint16_t *B, float *C)
+{
+    const int slots = b > 0 ? ell_cols / b : 0;
+    for (int r = 0; r < M; r++)
+        for (int n = 0; n < N; n++) {
+            float acc = 0.0f;
+            for (int j = 0; j < slots; j++) {
+                const int bc = col_ind[(size_t)(r / b) * slots + j];
+                if (bc < 0) continue;
+                for (int c = 0; c < b; c++) {
+                    const int k = bc * b + c;
+                    if (k >= K) break;
+                    const float a = orc_f16_to_f32(values[(size_t)r * ell_cols + j * b + c]);
+                    if (a != 0.0f) acc += a * orc_f16_to_f32(B[(size_t)k * N + n]);
+                }
+            }
+            C[(size_t)r * N + n] = acc;
+        }
+}

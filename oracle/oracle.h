@@ -24,6 +24,10 @@
  *   orc_tiling_*          -- CSR reconstruction identity + brute-force union.
  *   orc_*_e4m3 / _bf16    -- exhaustive decode tables vs the format definitions;
  *                            encode == nearest-even over all decoded values.
+ *   orc_dense_to_bell     -- hand-worked known answer, BELL -> dense identity,
+ *                            slot count == brute-force maximum, sorted columns.
+ *   orc_bell_spmm_f32     -- bit-equal to orc_spmm_f32 on the same matrix (same
+ *                            ascending-column FP32 order); -1 slots anywhere.
  */
 #ifndef TCA_ORACLE_H
 #define TCA_ORACLE_H
@@ -94,6 +98,22 @@ int orc_tiling_fill(int M, int K, const int *rowptr, const int *colidx, int rpw,
 /* CSR transpose (CSC): rowptrT[K+1], colidxT[nnz], perm[nnz] = source index. */
 void orc_transpose_csr(int M, int K, const int *rowptr, const int *colidx,
                        int *rowptrT, int *colidxT, int *perm);
+
+/* ---- Blocked-ELL: the host conversion of perform_cuSPARSE_BlockSpMM,
+ * src/spmm_cusparse.cu:254-331, and the product cusparseSpMM computes on it
+ * (:394-398).  Block rows = ceil(M/b); block (br, bc) is kept when one of its
+ * entries has (float)x != 0 (:258-266); slots per block row = the largest kept
+ * count (:281-291); col_ind[br*slots + j] = the j-th kept block column of block
+ * row br, ascending, -1 after the last (:293-311); values: (ceil(M/b)*b) x
+ * (slots*b) row-major, values[r*ell_cols + j*b + c] = A[r][col_ind*b + c], zero
+ * for -1 slots and past row M / column K (:313-331 read past K instead when K
+ * is not a multiple of b; the math is what is restated here).  Returns
+ * ell_cols = slots*b; col_ind / values may be NULL to size them. */
+int orc_dense_to_bell(const uint16_t *dense, int M, int K, int b, int32_t *col_ind, uint16_t *values);
+/* C[M x N] (f32) = BELL(A) * B: FP32 sums in ascending (slot, column-in-block)
+ * order, -1 slots skipped, B rows at or past K read as zero. */
+void orc_bell_spmm_f32(int M, int K, int N, int b, int ell_cols, const int32_t *col_ind,
+                       const uint16_t *values, const uint16_t *B, float *C);
 
 void orc_free(void *p);
 

@@ -76,6 +76,10 @@ def lib():
         L.orc_transpose_csr.restype = None
         L.orc_transpose_csr.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _i32p]
         L.orc_free.restype = None
+        L.orc_dense_to_bell.argtypes = [_u16p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _u16p]
+        L.orc_dense_to_bell.restype = ctypes.c_int
+        L.orc_bell_spmm_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
+                                        _u16p, _u16p, _f32p]
         L.orc_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
@@ -285,3 +289,34 @@ def transpose_csr(M, K, rowptr, col):
     colp = col if nnz else np.zeros(1, np.int32)
     lib().orc_transpose_csr(M, K, _p(rowptr, _i32p), _p(colp, _i32p), _p(rpT, _i32p), _p(cT, _i32p), _p(perm, _i32p))
     return rpT, cT[:nnz], perm[:nnz]
+
+
+def dense_to_bell(dense_bits, b):
+    """Blocked-ELL of a dense FP16 matrix (bits): (col_ind [ceil(M/b), slots] int32,
+    values [ceil(M/b)*b, slots*b] uint16, ell_cols)."""
+    d = np.ascontiguousarray(dense_bits, np.uint16)
+    M, K = d.shape
+    L = lib()
+    dp = d if d.size else np.zeros(1, np.uint16)
+    ell_cols = L.orc_dense_to_bell(_p(dp, _u16p), M, K, b, None, None)
+    BR, slots = (M + b - 1) // b, ell_cols // b
+    ci = np.zeros(max(BR * slots, 1), np.int32)
+    vals = np.zeros(max(BR * b * ell_cols, 1), np.uint16)
+    L.orc_dense_to_bell(_p(dp, _u16p), M, K, b, _p(ci, _i32p), _p(vals, _u16p))
+    return ci[:BR * slots].reshape(BR, slots), vals[:BR * b * ell_cols].reshape(BR * b, ell_cols), ell_cols
+
+
+def bell_spmm_f32(M, K, b, col_ind, values, B_bits):
+    """C = BELL(A) * B in FP32 (ascending column order)."""
+    B = np.ascontiguousarray(B_bits, np.uint16)
+    N = B.shape[1]
+    ci = np.ascontiguousarray(col_ind, np.int32).reshape(-1)
+    v = np.ascontiguousarray(values, np.uint16).reshape(-1)
+    ell_cols = values.shape[1] if values.ndim == 2 else 0
+    C = np.zeros((M, N), np.float32)
+    cip = ci if ci.size else np.zeros(1, np.int32)
+    vp = v if v.size else np.zeros(1, np.uint16)
+    Bp = B if B.size else np.zeros(1, np.uint16)
+    lib().orc_bell_spmm_f32(M, K, N, b, ell_cols, _p(cip, _i32p), _p(vp, _u16p), _p(Bp, _u16p),
+                            _p(C, _f32p) if C.size else None)
+    return C

@@ -350,6 +350,38 @@ extern "C" tca_status tca_pack_dense(const tca_plan *plan, tca_dtype dtype, cons
     return tca::launch_pack_dense(plan, dtype == TCA_E4M3 ? 1 : 2, d_tiled, d_dense, static_cast<hipStream_t>(stream));
 }
 
+extern "C" tca_status tca_bell_spmm(int n_rows, int n_cols, int block, int ell_cols, const int32_t *d_col_ind,
+                                    tca_dtype a_dtype, const void *d_values, tca_dtype b_dtype, const void *d_B, int ldb,
+                                    int N, float *d_C, int ldc, int tile_n, int rows_per_wave, void *stream)
+{
+    if (n_rows < 0 || n_cols < 0 || N <= 0) return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: bad shape");
+    if (block < 16 || block > 256 || (block & (block - 1)))
+        return fail(TCA_ERR_UNSUPPORTED, "tca_bell_spmm: block must be 16, 32, 64, 128 or 256");
+    if (ell_cols < 0 || ell_cols % block) return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: ell_cols must be a multiple of block");
+    if (N % 8) return fail(TCA_ERR_UNSUPPORTED, "tca_bell_spmm: N must be a multiple of 8");
+    if (ldb < N || ldc < N || ldb % 8 || ldc % 4) return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: ldb/ldc too small or misaligned");
+    if (!d_B || !d_C || (n_rows > 0 && ell_cols > 0 && (!d_col_ind || !d_values)))
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: NULL buffer");
+    if (!aligned(d_B, 16) || !aligned(d_C, 16) || !aligned(d_values, 16) || !aligned(d_col_ind, 4))
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: values, B and C must be 16-byte aligned");
+    if (tile_n != 0 && tile_n != 16 && tile_n != 32 && tile_n != 64)
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: tile_n must be 0, 16, 32 or 64");
+    if (rows_per_wave != 0 && rows_per_wave != 16 && rows_per_wave != 32 && rows_per_wave != 64)
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_spmm: rows_per_wave must be 0, 16, 32 or 64");
+    if (a_dtype != TCA_F16 && a_dtype != TCA_BF16) return fail(TCA_ERR_UNSUPPORTED, "tca_bell_spmm: values must be F16 or BF16");
+    if (b_dtype != a_dtype && b_dtype != TCA_F32) return fail(TCA_ERR_UNSUPPORTED, "tca_bell_spmm: B dtype must equal A dtype or be F32");
+    return tca::launch_bell_spmm(n_rows, n_cols, N, block, ell_cols, d_col_ind, a_dtype, d_values, b_dtype, d_B, ldb, d_C,
+                                 ldc, tile_n, rows_per_wave, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile)
+{
+    if (!rows_per_wave || !tile || block < 16 || (block & (block - 1)))
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_shape: bad argument");
+    tca::bell_shape(n_rows, N, block, tile_n, rows_per_wave, tile);
+    return TCA_OK;
+}
+
 extern "C" tca_status tca_sddmm(const tca_plan *plan, tca_dtype g_dtype, const void *d_G, int ldg, tca_dtype x_dtype,
                                 const void *d_X, int ldx, int N, float *d_out_tiled, void *stream)
 {

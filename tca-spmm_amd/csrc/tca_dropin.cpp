@@ -11,6 +11,7 @@
 #include <iterator>
 #include <sstream>
 #include <type_traits>
+#include <vector>
 
 namespace {
 
@@ -236,6 +237,61 @@ void perform_rocSPARSE_SpMM(half *A, half *B, float *C, int M, int N, int K, int
     HIP_CHECK(hipFree(drp));
     HIP_CHECK(hipFree(dci));
     HIP_CHECK(hipFree(dv));
+    HIP_CHECK(hipFree(dB));
+    HIP_CHECK(hipFree(dC));
+    destroy_csr(csr);
+}
+
+// src/spmm_cusparse.cu:181-445 (perform_cuSPARSE_BlockSpMM): dense A -> Blocked-ELL
+// on the host (untimed), then 5 warm-ups and num_iter timed tca_bell_spmm
+// launches; mean ms per launch in *elapsed.  C (M x N) is copied back whatever
+// M is (the reference skips the copy when M is not a multiple of the block).
+void perform_BlockSpMM(half *A, half *B, float *C, int M, int N, int K, int num_iter, int ell_block_size, double *elapsed)
+{
+    warm_up_gpu();
+    CSR *csr;
+    convert_dense_to_csr(A, &csr, M, K);
+    int ell_cols = 0;
+    const uint16_t *vals = reinterpret_cast<const uint16_t *>(csr->value);
+    TCA_CHECK(tca_bell_from_csr(M, K, csr->row_offset, csr->col_idx, vals, ell_block_size, &ell_cols, nullptr, nullptr));
+    const size_t ell_rows = ((size_t)M + ell_block_size - 1) / ell_block_size;
+    const size_t n_ind = ell_rows * (ell_cols / ell_block_size), n_val = ell_rows * ell_block_size * (size_t)ell_cols;
+    std::vector<int32_t> h_ind(n_ind + 1);
+    std::vector<uint16_t> h_val(n_val + 1);
+    TCA_CHECK(tca_bell_from_csr(M, K, csr->row_offset, csr->col_idx, vals, ell_block_size, &ell_cols, h_ind.data(),
+                                h_val.data()));
+    int32_t *d_ind;
+    uint16_t *d_val, *dB;
+    float *dC;
+    HIP_CHECK(hipMalloc(&d_ind, sizeof(int32_t) * (n_ind + 1)));
+    HIP_CHECK(hipMalloc(&d_val, sizeof(uint16_t) * (n_val + 1)));
+    HIP_CHECK(hipMalloc(&dB, sizeof(uint16_t) * (size_t)K * N));
+    HIP_CHECK(hipMalloc(&dC, sizeof(float) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(d_ind, h_ind.data(), sizeof(int32_t) * n_ind, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(d_val, h_val.data(), sizeof(uint16_t) * n_val, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dB, B, sizeof(uint16_t) * (size_t)K * N, hipMemcpyHostToDevice));
+    auto run = [&] {
+        TCA_CHECK(tca_bell_spmm(M, K, ell_block_size, ell_cols, d_ind, TCA_F16, d_val, TCA_F16, dB, N, N, dC, N, 0, 0,
+                                nullptr));
+    };
+    for (int i = 0; i < 5; i++) run();
+    HIP_CHECK(hipDeviceSynchronize());
+    hipEvent_t ev0, ev1;
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+    const int iters = num_iter > 0 ? num_iter : 1;
+    HIP_CHECK(hipEventRecord(ev0, nullptr));
+    for (int i = 0; i < iters; i++) run();
+    HIP_CHECK(hipEventRecord(ev1, nullptr));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    if (elapsed) *elapsed = (double)ms / iters;
+    HIP_CHECK(hipMemcpy(C, dC, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipEventDestroy(ev0));
+    HIP_CHECK(hipEventDestroy(ev1));
+    HIP_CHECK(hipFree(d_ind));
+    HIP_CHECK(hipFree(d_val));
     HIP_CHECK(hipFree(dB));
     HIP_CHECK(hipFree(dC));
     destroy_csr(csr);

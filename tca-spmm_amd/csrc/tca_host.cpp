@@ -320,6 +320,66 @@ extern "C" tca_status tca_csr_to_dense_f16(int n_rows, int n_cols, const int32_t
     return TCA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// CSR -> Blocked-ELL (the host conversion of perform_cuSPARSE_BlockSpMM,
+// src/spmm_cusparse.cu:254-331, from CSR instead of the dense matrix): a block
+// is kept when it holds an entry; each block row lists its kept block columns
+// ascending, -1 padded to the longest block row; values are the padded
+// (ceil(M/b)*b) x ell_cols row-major array, zeros off the pattern.
+// ---------------------------------------------------------------------------
+extern "C" tca_status tca_bell_from_csr(int n_rows, int n_cols, const int32_t *row_offset, const int32_t *col_idx,
+                                        const uint16_t *values, int block, int *ell_cols, int32_t *col_ind,
+                                        uint16_t *ell_values)
+{
+    if (n_rows < 0 || n_cols < 0 || !row_offset || !ell_cols || (row_offset[n_rows] > 0 && !col_idx))
+        return fail(TCA_ERR_INVALID_ARG, "tca_bell_from_csr: bad argument");
+    if (block < 16 || block > 256 || (block & (block - 1)))
+        return fail(TCA_ERR_UNSUPPORTED, "tca_bell_from_csr: block must be 16, 32, 64, 128 or 256");
+    const int BR = (n_rows + block - 1) / block, BC = (n_cols + block - 1) / block;
+    std::vector<int> stamp((size_t)std::max(BC, 1), -1), pos((size_t)std::max(BC, 1), 0);
+    std::vector<int> list;
+    int slots = 0;
+    for (int br = 0; br < BR; br++) {
+        int n = 0;
+        for (int r = br * block; r < std::min(n_rows, br * block + block); r++)
+            for (int e = row_offset[r]; e < row_offset[r + 1]; e++) {
+                const int c = col_idx[e];
+                if (c < 0 || c >= n_cols) return fail(TCA_ERR_FORMAT, "tca_bell_from_csr: column out of range");
+                if (stamp[c / block] != br) {
+                    stamp[c / block] = br;
+                    n++;
+                }
+            }
+        slots = std::max(slots, n);
+    }
+    if ((long)slots * block > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "tca_bell_from_csr: ell_cols exceeds int32");
+    *ell_cols = slots * block;
+    if (!col_ind || !ell_values) return TCA_OK;
+    const size_t width = (size_t)slots * block;
+    std::memset(ell_values, 0, sizeof(uint16_t) * (size_t)BR * block * width);
+    std::fill(stamp.begin(), stamp.end(), -1);
+    for (int br = 0; br < BR; br++) {
+        list.clear();
+        const int r1 = std::min(n_rows, br * block + block);
+        for (int r = br * block; r < r1; r++)
+            for (int e = row_offset[r]; e < row_offset[r + 1]; e++)
+                if (stamp[col_idx[e] / block] != br) {
+                    stamp[col_idx[e] / block] = br;
+                    list.push_back(col_idx[e] / block);
+                }
+        std::sort(list.begin(), list.end());
+        int32_t *ci = col_ind + (size_t)br * slots;
+        for (int j = 0; j < slots; j++) ci[j] = j < (int)list.size() ? list[j] : -1;
+        for (int j = 0; j < (int)list.size(); j++) pos[list[j]] = j;
+        for (int r = br * block; r < r1; r++)
+            for (int e = row_offset[r]; e < row_offset[r + 1]; e++) {
+                const int c = col_idx[e];
+                ell_values[(size_t)r * width + (size_t)pos[c / block] * block + c % block] = values ? values[e] : 0x3C00;
+            }
+    }
+    return TCA_OK;
+}
+
 // compare_mat<float> (src/general.cuh:71-123)
 extern "C" int tca_compare_mat_f32(const float *gold, const float *cmp, int n, float tol, int *n_fail)
 {

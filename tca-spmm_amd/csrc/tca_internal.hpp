@@ -109,6 +109,12 @@ tca_status sddmm_units(const tca_plan *aux, int cb, const int32_t **units, int *
 // operands with 32-bit byte offsets).
 uint32_t operand_bytes(long rows, long N, long ld, int elem_bytes);
 
+// Blocked-ELL SpMM (tca_bell.hip): shape choice and launch.
+void bell_shape(int M, int N, int block, int tile_n, int *rpw, int *nt);
+tca_status launch_bell_spmm(int M, int K, int N, int block, int ell_cols, const int32_t *col_ind, tca_dtype a_dtype,
+                            const void *values, tca_dtype b_dtype, const void *B, int ldb, float *C, int ldc,
+                            int tile_n, int rows_per_wave, hipStream_t stream);
+
 // Batched persistent launch: one descriptor per item, one record per work unit.
 struct BatchItemDev {
     int32_t variant;   // kernel variant id (batch_variant)

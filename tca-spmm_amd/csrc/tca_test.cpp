@@ -14,7 +14,7 @@
 int main(int argc, char *argv[])
 {
     if (argc <= 1) {
-        std::puts("Usage : ./tca_test {matrix.smtx} [N=256] [seed=0]");
+        std::puts("Usage : ./tca_test {matrix.smtx} [N=256] [seed=0] [ell_block=0: no Blocked-ELL]");
         std::exit(EXIT_FAILURE);
     }
     std::ifstream fin(argv[1], std::ifstream::in);
@@ -27,6 +27,7 @@ int main(int argc, char *argv[])
     smtx_input_to_Dense(fin, &A, argv[1], &M, &K, &nnz);
     const int N = argc > 2 ? std::atoi(argv[2]) : 256;
     const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 0;
+    const int ell_block = argc > 4 ? std::atoi(argv[4]) : 0;
     double e_gemm = -1.0, e_sparse = -1.0, e_our = -1.0;
     for (int i = 0; i < 10; i++) warm_up_gpu();
     std::printf("Input file - %s\n", argv[1]);
@@ -52,8 +53,18 @@ int main(int argc, char *argv[])
     convert_dense_to_csr(A, &csrA, M, K);
     std::memset(C_cmp, 0, sizeof(float) * (size_t)M * N);
     our_spmm_balanced(csrA, B, C_cmp, M, N, K, 10, &e_our);
-    const bool ok = compare_mat(C_gold, C_cmp, M * N, DEFAULT_TOL);
+    bool ok = compare_mat(C_gold, C_cmp, M * N, DEFAULT_TOL);
     if (!ok) std::printf("\033[35mvalidation failed (Our, balanced)\033[0m\n");
+
+    double e_bell = -1.0;
+    if (ell_block > 0) {   // the reference's block-SpMM baseline (not run by its driver)
+        std::memset(C_cmp, 0, sizeof(float) * (size_t)M * N);
+        perform_BlockSpMM(A, B, C_cmp, M, N, K, 10, ell_block, &e_bell);
+        if (!compare_mat(C_gold, C_cmp, M * N, DEFAULT_TOL)) {
+            std::printf("\033[35mvalidation failed (Blocked-ELL)\033[0m\n");
+            ok = false;
+        }
+    }
 
     std::free(A);
     destroy_csr(csrA);
@@ -63,5 +74,6 @@ int main(int argc, char *argv[])
     std::fprintf(stdout, "GemmEx : \033[36m[%lf] ms\033[0m\n", e_gemm);
     std::fprintf(stdout, "rocSPARSE: \033[36m[%lf] ms\033[0m \n", e_sparse);
     std::fprintf(stdout, "our: \033[36m[%lf] ms\033[0m \n", e_our);
+    if (ell_block > 0) std::fprintf(stdout, "Blocked-ELL(%d): \033[36m[%lf] ms\033[0m \n", ell_block, e_bell);
     return ok ? 0 : 1;
 }

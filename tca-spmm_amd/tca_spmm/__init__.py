@@ -5,7 +5,8 @@ src/our_spmm.cu our_spmm_balanced) over the C ABI of libtca_spmm.so.  Every
 computation runs in the HIP library; a missing library raises ImportError.
 """
 from ._lib import BF16, E4M3, F16, F32, TcaError, lib  # noqa: F401
-from . import autotune, gen, workloads  # noqa: F401
+from . import autotune, bell, gen, workloads  # noqa: F401
+from .bell import bell_from_csr, bell_shape  # noqa: F401
 
 lib()  # fail loudly at import when the native library is absent
 
@@ -14,8 +15,9 @@ try:  # torch-facing API (torch is the device-memory layer, not the compute)
                        quantize_cols_e4m3, quantize_rows_e4m3, rocsparse_spmm, sddmm, spmm)
     from .autograd import SparseLinear, SpmmFunction, sparse_matmul  # noqa: F401
     from .batch import Batch, sddmm_item, spmm_item  # noqa: F401
+    from .bell import BlockedEll, bell_spmm  # noqa: F401
 except ImportError:  # pragma: no cover - torch absent
     pass
 
 __all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
-           "hipblaslt_gemm", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "TcaError", "F16", "BF16", "E4M3", "F32"]
+           "hipblaslt_gemm", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "TcaError", "F16", "BF16", "E4M3", "F32"]
