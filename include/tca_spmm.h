@@ -285,6 +285,17 @@ tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row
 /* hipBLASLt dense GEMM C = A_dense * B, f16 in, f32 out (row-major). */
 tca_status tca_hipblaslt_gemm(int M, int N, int K, const uint16_t *d_A, const uint16_t *d_B, float *d_C,
                               int n_iter, double *ms_per_iter);
+/* 2:4 structured sparsity on hipSPARSELt.  Replaces perform_cuSPARSELT_24sparsity
+ * (src/spmm_cusparse.cu:2-179, decl src/spmm_cusparse.cuh:13): A (dense row-major
+ * M x K, F16) pruned with the STRIP rule (per 1x4 strip of a row the pair of
+ * largest L1 norm survives) into d_A_pruned (M x K, may be NULL), compressed,
+ * and C (row-major M x N, F16, FP32 accumulation) = pruned A * B (K x N, F16).
+ * A different product from tca_spmm's (A loses entries): a speed comparison.
+ * M, N, K multiples of 8.  TCA_ERR_UNSUPPORTED when the process has bound a
+ * hipSPARSELt other than the one the library was built against (PyTorch bundles
+ * an older copy; use a process without it, e.g. tca_test --sparse24). */
+tca_status tca_hipsparselt_spmm_24(int M, int N, int K, const uint16_t *d_A, uint16_t *d_A_pruned,
+                                   const uint16_t *d_B, uint16_t *d_C, int n_iter, double *ms_per_iter);
 
 #ifdef __cplusplus
 }

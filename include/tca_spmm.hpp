@@ -41,6 +41,9 @@ void perform_hipBLASLt_GemmEx(half *A, half *B, float *C, int M, int N, int K, i
  * with blocks of ell_block_size (16/32/64/128/256), then the gfx950 Blocked-ELL
  * kernel (tca_bell_spmm); C (M x N) is overwritten. */
 void perform_BlockSpMM(half *A, half *B, float *C, int M, int N, int K, int num_iter, int ell_block_size, double *elapsed);
+/* src/spmm_cusparse.cuh:13 (perform_cuSPARSELT_24sparsity) on hipSPARSELt: A pruned to
+ * 2:4 (STRIP), compressed, multiplied; the F16 result is widened into C. */
+void perform_hipSPARSELt_24sparsity(half *A, half *B, float *C, int M, int N, int K, int num_iter, double *elapsed);
 /* src/miscalleneous.cu:158-163 (warm_up_gpu_kernel<<<1,32>>>) */
 void warm_up_gpu();
 

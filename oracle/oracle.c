@@ -575,3 +575,26 @@ void orc_bell_spmm_f32(int M, int K, int N, int b, int ell_cols, const int32_t *
             C[(size_t)r * N + n] = acc;
         }
 }
+
+/* cusparseLtSpMMAPrune, STRIP rule (src/spmm_cusparse.cu:95-96) */
+int orc_prune_24_strip(const uint16_t *A, int M, int K, uint16_t *out)
+{
+    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    if (K % 4) return -1;
+    for (int r = 0; r < M; r++)
+        for (int g = 0; g < K; g += 4) {
+            const uint16_t *a = A + (size_t)r * K + g;
+            uint16_t *o = out + (size_t)r * K + g;
+            float mag[4];
+            for (int i = 0; i < 4; i++) mag[i] = fabsf(orc_f16_to_f32(a[i]));
+            int best = 0;
+            float bestv = mag[pa[0]] + mag[pb[0]];
+            for (int c = 1; c < 6; c++)
+                if (mag[pa[c]] + mag[pb[c]] > bestv) {
+                    bestv = mag[pa[c]] + mag[pb[c]];
+                    best = c;
+                }
+            for (int i = 0; i < 4; i++) o[i] = (i == pa[best] || i == pb[best]) ? a[i] : 0;
+        }
+    return 0;
+}

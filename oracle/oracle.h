@@ -28,6 +28,8 @@
  *                            slot count == brute-force maximum, sorted columns.
  *   orc_bell_spmm_f32     -- bit-equal to orc_spmm_f32 on the same matrix (same
  *                            ascending-column FP32 order); -1 slots anywhere.
+ *   orc_prune_24_strip    -- hand-worked strips, brute force over all pairs on
+ *                            tie-free rows, idempotence, <= 2 survivors per strip.
  */
 #ifndef TCA_ORACLE_H
 #define TCA_ORACLE_H
@@ -114,6 +116,16 @@ int orc_dense_to_bell(const uint16_t *dense, int M, int K, int b, int32_t *col_i
  * order, -1 slots skipped, B rows at or past K read as zero. */
 void orc_bell_spmm_f32(int M, int K, int N, int b, int ell_cols, const int32_t *col_ind,
                        const uint16_t *values, const uint16_t *B, float *C);
+
+/* ---- 2:4 pruning: cusparseLtSpMMAPrune(..., CUSPARSELT_PRUNE_SPMMA_STRIP) as
+ * perform_cuSPARSELT_24sparsity calls it (src/spmm_cusparse.cu:95-96), restated
+ * from the library's published rule (cuSPARSELt is not in the reference; the
+ * hipSPARSELt header words it the same way): in every 1x4 strip of a row the
+ * pair of largest L1 norm survives, the other two become +0.  The order in
+ * which the library breaks ties is not published: here the first pair in
+ * (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) order wins (parity unpinned on ties).
+ * K % 4 == 0; returns -1 otherwise. */
+int orc_prune_24_strip(const uint16_t *A, int M, int K, uint16_t *out);
 
 void orc_free(void *p);
 

@@ -80,6 +80,8 @@ def lib():
         L.orc_dense_to_bell.restype = ctypes.c_int
         L.orc_bell_spmm_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
                                         _u16p, _u16p, _f32p]
+        L.orc_prune_24_strip.argtypes = [_u16p, ctypes.c_int, ctypes.c_int, _u16p]
+        L.orc_prune_24_strip.restype = ctypes.c_int
         L.orc_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
@@ -320,3 +322,15 @@ def bell_spmm_f32(M, K, b, col_ind, values, B_bits):
     lib().orc_bell_spmm_f32(M, K, N, b, ell_cols, _p(cip, _i32p), _p(vp, _u16p), _p(Bp, _u16p),
                             _p(C, _f32p) if C.size else None)
     return C
+
+
+def prune_24_strip(A_bits):
+    """2:4 STRIP pruning of a dense FP16 matrix (bits), rows of K % 4 == 0."""
+    A = np.ascontiguousarray(A_bits, np.uint16)
+    M, K = A.shape
+    out = np.zeros_like(A)
+    if A.size:
+        rc = lib().orc_prune_24_strip(_p(A, _u16p), M, K, _p(out, _u16p))
+        if rc != 0:
+            raise ValueError("K must be a multiple of 4")
+    return out

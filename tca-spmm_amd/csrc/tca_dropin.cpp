@@ -297,6 +297,32 @@ void perform_BlockSpMM(half *A, half *B, float *C, int M, int N, int K, int num_
     destroy_csr(csr);
 }
 
+// src/spmm_cusparse.cu:2-179 on hipSPARSELt: the F16 product is widened into C
+// like the reference's __half2float loop (:158-161).
+void perform_hipSPARSELt_24sparsity(half *A, half *B, float *C, int M, int N, int K, int num_iter, double *elapsed)
+{
+    warm_up_gpu();
+    uint16_t *dA, *dB, *dC;
+    HIP_CHECK(hipMalloc(&dA, sizeof(uint16_t) * (size_t)M * K));
+    HIP_CHECK(hipMalloc(&dB, sizeof(uint16_t) * (size_t)K * N));
+    HIP_CHECK(hipMalloc(&dC, sizeof(uint16_t) * (size_t)M * N));
+    HIP_CHECK(hipMemcpy(dA, A, sizeof(uint16_t) * (size_t)M * K, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dB, B, sizeof(uint16_t) * (size_t)K * N, hipMemcpyHostToDevice));
+    double ms = 0.0;
+    TCA_CHECK(tca_hipsparselt_spmm_24(M, N, K, dA, nullptr, dB, dC, num_iter > 0 ? num_iter : 1, &ms));
+    if (elapsed) *elapsed = ms;
+    std::vector<uint16_t> hC((size_t)M * N);
+    HIP_CHECK(hipMemcpy(hC.data(), dC, sizeof(uint16_t) * hC.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < hC.size(); i++) {
+        half h;
+        std::memcpy(&h, &hC[i], sizeof(h));
+        C[i] = __half2float(h);
+    }
+    HIP_CHECK(hipFree(dA));
+    HIP_CHECK(hipFree(dB));
+    HIP_CHECK(hipFree(dC));
+}
+
 // src/gemm_cublas.cu:3-111 on hipBLASLt
 void perform_hipBLASLt_GemmEx(half *A, half *B, float *C, int M, int N, int K, int n_iter, double *elapsed)
 {

@@ -5,14 +5,75 @@
 // against it with compare_mat and prints the three timings.
 #include "../../include/tca_spmm.hpp"
 
+#include <hip/hip_runtime.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <string>
 #include <vector>
+
+// --sparse24 M N K n_iter dir: the hipSPARSELt 2:4 baseline in a process of its own
+// (no other hipSPARSELt loaded).  Reads dir/A.bin (M x K) and dir/B.bin (K x N), F16
+// row-major; writes dir/P.bin (pruned A) and dir/C.bin (M x N, F16); prints the mean
+// ms per call over n_iter calls.
+static int run_sparse24(int argc, char *argv[])
+{
+    if (argc < 7) {
+        std::puts("Usage : ./tca_test --sparse24 M N K n_iter dir");
+        return 2;
+    }
+    const int M = std::atoi(argv[2]), N = std::atoi(argv[3]), K = std::atoi(argv[4]), n_iter = std::atoi(argv[5]);
+    const std::string dir = argv[6];
+    if (M <= 0 || N <= 0 || K <= 0) return 2;
+    std::vector<uint16_t> A((size_t)M * K), B((size_t)K * N), P(A.size()), C((size_t)M * N);
+    auto rd = [&](const char *name, std::vector<uint16_t> &v) {
+        FILE *f = std::fopen((dir + "/" + name).c_str(), "rb");
+        if (!f) return false;
+        const size_t n = std::fread(v.data(), sizeof(uint16_t), v.size(), f);
+        std::fclose(f);
+        return n == v.size();
+    };
+    auto wr = [&](const char *name, const std::vector<uint16_t> &v) {
+        FILE *f = std::fopen((dir + "/" + name).c_str(), "wb");
+        if (!f) return false;
+        const size_t n = std::fwrite(v.data(), sizeof(uint16_t), v.size(), f);
+        std::fclose(f);
+        return n == v.size();
+    };
+    if (!rd("A.bin", A) || !rd("B.bin", B)) {
+        std::fprintf(stderr, "cannot read %s/A.bin or B.bin\n", dir.c_str());
+        return 2;
+    }
+    uint16_t *dA, *dB, *dP, *dC;
+    if (hipMalloc(&dA, A.size() * 2) != hipSuccess || hipMalloc(&dB, B.size() * 2) != hipSuccess ||
+        hipMalloc(&dP, P.size() * 2) != hipSuccess || hipMalloc(&dC, C.size() * 2) != hipSuccess) {
+        std::fprintf(stderr, "hipMalloc failed\n");
+        return 1;
+    }
+    (void)hipMemcpy(dA, A.data(), A.size() * 2, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dB, B.data(), B.size() * 2, hipMemcpyHostToDevice);
+    double ms = 0.0;
+    const tca_status s = tca_hipsparselt_spmm_24(M, N, K, dA, dP, dB, dC, n_iter, &ms);
+    if (s != TCA_OK) {
+        std::fprintf(stderr, "tca_hipsparselt_spmm_24: %s (%s)\n", tca_status_string(s), tca_last_error());
+        return 1;
+    }
+    (void)hipMemcpy(P.data(), dP, P.size() * 2, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(C.data(), dC, C.size() * 2, hipMemcpyDeviceToHost);
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dP);
+    (void)hipFree(dC);
+    if (!wr("P.bin", P) || !wr("C.bin", C)) return 1;
+    std::printf("hipSPARSELt 2:4 ms %f\n", ms);
+    return 0;
+}
 
 int main(int argc, char *argv[])
 {
+    if (argc > 1 && std::strcmp(argv[1], "--sparse24") == 0) return run_sparse24(argc, argv);
     if (argc <= 1) {
         std::puts("Usage : ./tca_test {matrix.smtx} [N=256] [seed=0] [ell_block=0: no Blocked-ELL]");
         std::exit(EXIT_FAILURE);

@@ -7,6 +7,7 @@ computation runs in the HIP library; a missing library raises ImportError.
 from ._lib import BF16, E4M3, F16, F32, TcaError, lib  # noqa: F401
 from . import autotune, bell, gen, workloads  # noqa: F401
 from .bell import bell_from_csr, bell_shape  # noqa: F401
+from .sparse24 import hipsparselt_spmm_24  # noqa: F401
 
 lib()  # fail loudly at import when the native library is absent
 
@@ -20,4 +21,4 @@ except ImportError:  # pragma: no cover - torch absent
     pass
 
 __all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
-           "hipblaslt_gemm", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "TcaError", "F16", "BF16", "E4M3", "F32"]
+           "hipblaslt_gemm", "hipsparselt_spmm_24", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "TcaError", "F16", "BF16", "E4M3", "F32"]

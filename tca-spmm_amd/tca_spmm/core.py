@@ -270,3 +270,4 @@ def hipblaslt_gemm(M, N, K, A_dense, B, out, n_iter=0):
     ms = ctypes.c_double(0.0)
     call("tca_hipblaslt_gemm", M, N, K, _ptr(A_dense), _ptr(B), _ptr(out), n_iter, ctypes.byref(ms))
     return ms.value
+
