@@ -439,6 +439,26 @@ def main():
         bell["ms_per_step"][f"b{b}"] = tot * 1e-3
     extra["blocked_ell"] = bell
 
+    # 2:4 (the reference's cuSPARSELt comparison, here gfx950's sparse MFMA): each
+    # product's A densified, pruned to 2:4 and compressed outside the timed
+    # region -- a different, denser product than the SpMM's
+    s24 = {"rows": []}
+    tot = 0.0
+    for e in entries:
+        A = e["case"].A
+        dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
+        rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
+        dense[rows, torch.from_numpy(A.col_idx).to(dev).long()] = f16(A.values)
+        sp = T.Sparse24.compress(dense)
+        C24 = torch.empty_like(e["C"])
+        us = time_eager(torch, lambda: T.spmm_24(sp, e["B"], out=C24), 50, backlog=True) * 1e3
+        kept = int((sp.values != 0).sum())
+        s24["rows"].append({"name": A.name, "us": us, "kept_nnz_frac": kept / max(A.nnz, 1)})
+        tot += us
+        del dense, sp
+    s24["ms_per_step"] = tot * 1e-3
+    extra["sparse24"] = s24
+
     # achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
     src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
     dst = torch.empty_like(src)

@@ -223,6 +223,28 @@ tca_status tca_bell_spmm(int n_rows, int n_cols, int block, int ell_cols, const 
 /* The (rows_per_wave, tile_n) tca_bell_spmm picks for a shape when given 0s. */
 tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile);
 
+/* ------------------------------------------------------------------------ */
+/* 2:4 structured sparsity on gfx950's sparse MFMA.  Replaces                 */
+/* perform_cuSPARSELT_24sparsity (src/spmm_cusparse.cu:2-179, decl            */
+/* src/spmm_cusparse.cuh:13): prune + compress (cusparseLtSpMMAPrune STRIP,   */
+/* cusparseLtSpMMACompress, :95-136) and the product (cusparseLtMatmul,       */
+/* :145-148; FP32 C here instead of the reference's FP16).  A different       */
+/* product from tca_spmm's: A loses entries.                                 */
+/* ------------------------------------------------------------------------ */
+/* Prune a dense row-major M x K (ld) F16/BF16 matrix with the STRIP rule (per
+ * 1x4 strip of a row the pair of largest L1 norm survives; ties to the first
+ * pair of (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)) and compress it: values M x K/2
+ * (the survivors in column order), meta M x K/16 uint16 (value j of a
+ * 16-column block has its column-in-strip at bits 2j+1:2j).  K % 64 == 0. */
+tca_status tca_compress_24(int n_rows, int n_cols, tca_dtype dtype, const void *d_dense, int ld,
+                           void *d_values, uint16_t *d_meta, void *stream);
+/* C[M x N] (f32, every row written) = compressed 2:4 A (M x K) * B (K x N
+ * row-major, the values' dtype or F32 rounded on load).  K % 64 == 0, N % 8 == 0;
+ * values, B, C 16-byte aligned.  tile_n / rows_per_wave: 0 = auto, 16/32/64. */
+tca_status tca_spmm_24(int n_rows, int n_cols, int N, tca_dtype a_dtype, const void *d_values,
+                       const uint16_t *d_meta, tca_dtype b_dtype, const void *d_B, int ldb, float *d_C,
+                       int ldc, int tile_n, int rows_per_wave, void *stream);
+
 /* SDDMM on the plan's pattern: out_tiled[e] = sum_n G[row(e), n] * X[col(e), n]
  * (the dW of Y = W X with dY = G).  G: M x N row-major (ldg), X: K x N (ldx),
  * dtypes F16/BF16, or F32 (G) rounded to X's type on load.  N % 32 == 0.

@@ -374,6 +374,40 @@ extern "C" tca_status tca_bell_spmm(int n_rows, int n_cols, int block, int ell_c
                                  ldc, tile_n, rows_per_wave, static_cast<hipStream_t>(stream));
 }
 
+extern "C" tca_status tca_compress_24(int n_rows, int n_cols, tca_dtype dtype, const void *d_dense, int ld,
+                                      void *d_values, uint16_t *d_meta, void *stream)
+{
+    if (n_rows < 0 || n_cols < 0 || ld < n_cols || ld % 8) return fail(TCA_ERR_INVALID_ARG, "tca_compress_24: bad shape or ld");
+    if (n_cols % 64) return fail(TCA_ERR_UNSUPPORTED, "tca_compress_24: n_cols must be a multiple of 64");
+    if (dtype != TCA_F16 && dtype != TCA_BF16) return fail(TCA_ERR_UNSUPPORTED, "tca_compress_24: dtype must be F16 or BF16");
+    if ((long)n_rows * n_cols > 0 && (!d_dense || !d_values || !d_meta)) return fail(TCA_ERR_INVALID_ARG, "tca_compress_24: NULL buffer");
+    if (!aligned(d_dense, 16) || !aligned(d_values, 16) || !aligned(d_meta, 2))
+        return fail(TCA_ERR_INVALID_ARG, "tca_compress_24: dense and values must be 16-byte aligned");
+    return tca::launch_compress24(n_rows, n_cols, dtype, d_dense, ld, d_values, d_meta, static_cast<hipStream_t>(stream));
+}
+
+extern "C" tca_status tca_spmm_24(int n_rows, int n_cols, int N, tca_dtype a_dtype, const void *d_values,
+                                  const uint16_t *d_meta, tca_dtype b_dtype, const void *d_B, int ldb, float *d_C,
+                                  int ldc, int tile_n, int rows_per_wave, void *stream)
+{
+    if (n_rows < 0 || n_cols < 0 || N <= 0) return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: bad shape");
+    if (n_cols % 64) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_24: n_cols must be a multiple of 64");
+    if (N % 8) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_24: N must be a multiple of 8");
+    if (ldb < N || ldc < N || ldb % 8 || ldc % 4) return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: ldb/ldc too small or misaligned");
+    if (!d_B || !d_C || ((long)n_rows * n_cols > 0 && (!d_values || !d_meta)))
+        return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: NULL buffer");
+    if (!aligned(d_B, 16) || !aligned(d_C, 16) || !aligned(d_values, 16) || !aligned(d_meta, 2))
+        return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: values, B and C must be 16-byte aligned");
+    if (tile_n != 0 && tile_n != 16 && tile_n != 32 && tile_n != 64)
+        return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: tile_n must be 0, 16, 32 or 64");
+    if (rows_per_wave != 0 && rows_per_wave != 16 && rows_per_wave != 32 && rows_per_wave != 64)
+        return fail(TCA_ERR_INVALID_ARG, "tca_spmm_24: rows_per_wave must be 0, 16, 32 or 64");
+    if (a_dtype != TCA_F16 && a_dtype != TCA_BF16) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_24: values must be F16 or BF16");
+    if (b_dtype != a_dtype && b_dtype != TCA_F32) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_24: B dtype must equal A dtype or be F32");
+    return tca::launch_spmm24(n_rows, n_cols, N, a_dtype, d_values, d_meta, b_dtype, d_B, ldb, d_C, ldc, tile_n,
+                              rows_per_wave, static_cast<hipStream_t>(stream));
+}
+
 extern "C" tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile)
 {
     if (!rows_per_wave || !tile || block < 16 || (block & (block - 1)))

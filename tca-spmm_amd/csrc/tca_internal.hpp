@@ -111,6 +111,12 @@ uint32_t operand_bytes(long rows, long N, long ld, int elem_bytes);
 
 // Blocked-ELL SpMM (tca_bell.hip): shape choice and launch.
 void bell_shape(int M, int N, int block, int tile_n, int *rpw, int *nt);
+// 2:4 sparse MFMA path (tca_sparse24.hip).
+tca_status launch_compress24(int M, int K, tca_dtype dtype, const void *dense, int ld, void *values, uint16_t *meta,
+                             hipStream_t stream);
+tca_status launch_spmm24(int M, int K, int N, tca_dtype a_dtype, const void *values, const uint16_t *meta,
+                         tca_dtype b_dtype, const void *B, int ldb, float *C, int ldc, int tile_n, int rows_per_wave,
+                         hipStream_t stream);
 tca_status launch_bell_spmm(int M, int K, int N, int block, int ell_cols, const int32_t *col_ind, tca_dtype a_dtype,
                             const void *values, tca_dtype b_dtype, const void *B, int ldb, float *C, int ldc,
                             int tile_n, int rows_per_wave, hipStream_t stream);
