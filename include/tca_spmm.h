@@ -271,6 +271,9 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 /* transposed, with an F16 B on 16-row (tile_n 16/32/64), 32-row (32/64) or  */
 /* 64-row (32) plans, or an F32 B on 16-row (16/32/64) or 32-row (32) plans. */
 /* SDDMM items: F16 X, F32 or F16 G.  Other combinations: UNSUPPORTED.       */
+/* An F32 operand (SpMM B, SDDMM G) gets a batch-owned F16 copy (2 bytes per  */
+/* element) that every run refreshes first -- one extra kernel, same bits as  */
+/* rounding on load -- and the items read the copy: half the gather bytes.   */
 /* ------------------------------------------------------------------------ */
 typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
 typedef struct tca_batch_item {

@@ -18,6 +18,13 @@ struct tca_batch {
     int *d_counters = nullptr;
     int *d_part_off = nullptr;   // first unit of each partition (kBatchParts + 1)
     int nparts = 1;
+    // FP32 operands (SpMM B, SDDMM G) rounded once per run into FP16 copies that
+    // the items read instead: the gathers then move half the bytes, with the
+    // same bits as rounding on load
+    void *d_shadow = nullptr;    // conversion descriptors | FP16 copies
+    tca::ConvDesc *d_conv = nullptr;
+    int n_conv = 0;
+    long conv8 = 0;
 };
 
 namespace {
@@ -30,6 +37,84 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
 {
     if (!batch || n_items <= 0 || !items) return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: bad argument");
     *batch = nullptr;
+    // FP32 operands read by several items or one: one FP16 copy per buffer, used
+    // when every item reading it agrees on its shape
+    std::vector<tca_batch_item> its(items, items + n_items);
+    struct Operand {
+        const void *src;
+        int rows, N, ld;
+        bool ok;
+        size_t off;
+    };
+    std::vector<Operand> ops;
+    auto operand_of = [&](int i, bool is_b) -> std::pair<const void *, Operand> {
+        const tca_batch_item &it = its[i];
+        const tca_plan *p = it.plan;
+        if (!p) return {nullptr, {}};
+        if (it.op == TCA_OP_SPMM && is_b && it.b_dtype == TCA_F32 && it.b)
+            return {it.b, {it.b, p->host.n_cols, it.N, it.ldb, true, 0}};
+        if (it.op == TCA_OP_SDDMM && !is_b && it.a_dtype == TCA_F32 && it.a)
+            return {it.a, {it.a, p->host.n_rows, it.N, it.lda, true, 0}};
+        return {nullptr, {}};
+    };
+    for (int i = 0; i < n_items; i++)
+        for (bool is_b : {true, false}) {
+            const auto found = operand_of(i, is_b);
+            const void *ptr = found.first;
+            const Operand &o = found.second;
+            if (!ptr || o.N <= 0 || o.N % 8 || o.ld < o.N || o.ld % 8 || o.rows <= 0) continue;
+            auto f = std::find_if(ops.begin(), ops.end(), [&](const Operand &x) { return x.src == ptr; });
+            if (f == ops.end()) ops.push_back(o);
+            else if (f->rows != o.rows || f->N != o.N || f->ld != o.ld) f->ok = false;
+        }
+    ops.erase(std::remove_if(ops.begin(), ops.end(), [](const Operand &x) { return !x.ok; }), ops.end());
+    const size_t conv_bytes = (sizeof(tca::ConvDesc) * std::max<size_t>(ops.size(), 1) + 255) / 256 * 256;
+    size_t shadow_bytes = conv_bytes;
+    for (Operand &o : ops) {
+        o.off = shadow_bytes;
+        shadow_bytes += ((size_t)o.rows * o.N * 2 + 255) / 256 * 256;
+    }
+    struct DevGuard {
+        void *p = nullptr;
+        ~DevGuard()
+        {
+            if (p) (void)hipFree(p);
+        }
+    } shadow;
+    std::vector<tca::ConvDesc> conv;
+    long conv8 = 0;
+    if (!ops.empty()) {
+        if (hipMalloc(&shadow.p, shadow_bytes) != hipSuccess) return fail(TCA_ERR_ALLOC, "tca_batch_create: FP16 copies");
+        char *base = static_cast<char *>(shadow.p);
+        for (const Operand &o : ops) {
+            tca::ConvDesc c{};
+            c.src = static_cast<const float *>(o.src);
+            c.dst = reinterpret_cast<uint16_t *>(base + o.off);
+            c.rows = o.rows;
+            c.N = o.N;
+            c.ld = o.ld;
+            c.first8 = conv8;
+            conv8 += ((long)o.rows * o.N / 8 + 255) / 256 * 256;   // next operand on a workgroup boundary
+            conv.push_back(c);
+        }
+        for (tca_batch_item &it : its) {
+            for (const tca::ConvDesc &c : conv) {
+                if (it.op == TCA_OP_SPMM && it.b_dtype == TCA_F32 && it.b == c.src) {
+                    it.b = c.dst;
+                    it.ldb = c.N;
+                    it.b_dtype = TCA_F16;
+                }
+                if (it.op == TCA_OP_SDDMM && it.a_dtype == TCA_F32 && it.a == c.src) {
+                    it.a = c.dst;
+                    it.lda = c.N;
+                    it.a_dtype = TCA_F16;
+                }
+            }
+        }
+        if (hipMemcpy(shadow.p, conv.data(), sizeof(tca::ConvDesc) * conv.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
+        items = its.data();
+    }
     std::vector<tca::BatchItemDev> dev(n_items);
     std::vector<tca::BatchUnitDev> units;
     std::vector<double> cost;
@@ -229,6 +314,11 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     b->dense_only = dense_only;
     b->grid = grid;
     b->nparts = nparts;
+    b->n_conv = (int)conv.size();
+    b->conv8 = conv8;
+    b->d_conv = static_cast<tca::ConvDesc *>(shadow.p);
+    b->d_shadow = shadow.p;
+    shadow.p = nullptr;   // owned by the batch from here
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
     const size_t bz = sizeof(tca::ZeroDesc) * std::max<size_t>(zero.size(), 1);
@@ -258,6 +348,10 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
 extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
 {
     if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
+    if (batch->n_conv) {
+        const tca_status s = tca::launch_convert_f16(batch->d_conv, batch->n_conv, batch->conv8, static_cast<hipStream_t>(stream));
+        if (s != TCA_OK) return s;
+    }
     return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->d_part_off,
                              batch->nparts, batch->grid, batch->d_zero, batch->n_zero, batch->dense_only,
                              static_cast<hipStream_t>(stream));
@@ -267,6 +361,7 @@ extern "C" tca_status tca_batch_destroy(tca_batch *batch)
 {
     if (!batch) return TCA_OK;
     if (batch->d_mem) (void)hipFree(batch->d_mem);
+    if (batch->d_shadow) (void)hipFree(batch->d_shadow);
     delete batch;
     return TCA_OK;
 }

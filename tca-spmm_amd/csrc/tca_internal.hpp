@@ -142,6 +142,15 @@ struct ZeroDesc {              // an output that split units accumulate into
     float *C;
     int32_t rows, N, ldc, pad;
 };
+struct ConvDesc {              // an FP32 operand a batch rounds to an FP16 copy on every run
+    const float *src;
+    uint16_t *dst;             // rows x N, row pitch N
+    int32_t rows, N, ld, pad;
+    int64_t first8;            // its first 8-element group in the conversion launch (a multiple of 256)
+};
+// Rounds every operand of descs (nearest even, the rounding the kernels apply
+// on load) into its FP16 copy: one launch over total8 groups of 8 elements.
+tca_status launch_convert_f16(const ConvDesc *descs, int n, long total8, hipStream_t stream);
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // Work counters of a batch launch: the unit list is cut into up to
 // kBatchParts partitions (one per XCD under round-robin workgroup dispatch),

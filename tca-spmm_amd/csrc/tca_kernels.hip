@@ -1678,6 +1678,43 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     return TCA_OK;
 }
 
+namespace {
+
+// One thread per 8 consecutive elements of one row: two 16-byte loads, one
+// 16-byte store.  Every operand starts on a 256-group boundary (first8), so a
+// workgroup's operand is uniform: found by a scalar scan of the (few)
+// descriptors.
+__global__ __launch_bounds__(256) void convert_f16_kernel(const ConvDesc *descs, int n, long total8)
+{
+    for (long blk = blockIdx.x; blk * 256 < total8; blk += gridDim.x) {
+        const long t0 = blk * 256;
+        int k = 0;
+        while (k + 1 < n && descs[k + 1].first8 <= t0) k++;
+        const float *src0 = descs[k].src;
+        uint16_t *dst0 = descs[k].dst;
+        const int N = descs[k].N, ld = descs[k].ld;
+        const long e = (t0 + threadIdx.x - descs[k].first8) * 8;
+        if (e >= (long)descs[k].rows * N) continue;   // this operand's padding
+        const long r = e / N, col = e - r * N;
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(src0 + r * ld + col);
+        const u32x4 a = gbl(src)[0], b = gbl(src)[1];
+        auto h = [](uint32_t x) { return f32_to_f16_bits(__builtin_bit_cast(float, x)); };
+        const u32x4 o = {h(a[0]) | h(a[1]) << 16, h(a[2]) | h(a[3]) << 16, h(b[0]) | h(b[1]) << 16, h(b[2]) | h(b[3]) << 16};
+        gbl(reinterpret_cast<u32x4 *>(dst0 + r * N + col))[0] = o;
+    }
+}
+
+}  // namespace
+
+tca_status launch_convert_f16(const ConvDesc *descs, int n, long total8, hipStream_t stream)
+{
+    if (n <= 0 || total8 <= 0) return TCA_OK;
+    const long blocks = std::min<long>((total8 + 255) / 256, 8192);
+    hipLaunchKernelGGL(convert_f16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, descs, n, total8);
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
 tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *src, void *dst, bool scatter,
                          hipStream_t stream)
 {

@@ -126,3 +126,21 @@ def test_batch_with_fp16_gradient_matches_fp32_gradient(cuda):
     torch.cuda.synchronize()
     for L, (dx, dw) in zip(layers, ref):
         assert torch.equal(L["dX"], dx) and torch.equal(L["dW"], dw)
+
+
+def test_batch_rounds_fp32_operands_on_every_run(cuda):
+    # FP32 G is rounded into the batch's FP16 copy at the start of every run, so
+    # a G rewritten between runs is what the next run reads
+    layers = _layers(cuda)[:4]
+    batch = T.Batch(_items(layers))
+    batch.run()
+    torch.cuda.synchronize()
+    for L in layers:
+        L["G"].mul_(-0.75).add_(0.125)
+    batch.run()
+    torch.cuda.synchronize()
+    for L in layers:
+        dW = T.sddmm(L["plan"], L["G"], L["X"])
+        dX = T.spmm(L["plan_t"], L["w"], L["G"], tile_n=L["tile_t"])
+        assert torch.equal(L["dW"], dW)
+        assert rel_l2(L["dX"].cpu().numpy(), dX.cpu().numpy()) <= 1e-6
