@@ -159,6 +159,17 @@ def bench_fwd_bwd(torch, np, T, O, dev):
             L["b"]["plan_t"].pack_dense(L["w16"], out=L["b"]["at"])
 
     bt = T.Batch(items)
+    # parity gate: the fused launch against the per-call dense-tile launches
+    bt.run()
+    torch.cuda.synchronize()
+    fused_out = [(L["Y"].clone(), L["dX"].clone(), L["dW"].clone()) for L in layers]
+    per_call("d")()
+    torch.cuda.synchronize()
+    for L, (y, dx, dw) in zip(layers, fused_out):
+        for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["dW"])):
+            rel = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
+            if rel > 1e-6:
+                raise RuntimeError(f"fused forward + backward differs from the per-call launches (rel-L2 {rel:.2e})")
     plain_eager = time_eager(torch, per_call("t"), 20)
     plain_graph = time_graph(torch, per_call("t"), 20)
     dense_graph = time_graph(torch, per_call("d"), 20)

@@ -58,14 +58,16 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False, dense=False):
     parallelism comes from the many items, and every work unit pays a fixed
     claim + pipeline-start cost, so few, long units win: tiled values on
     16 rows x 64 columns; panel-dense tiles on 32 rows (each gathered B row
-    serves two MFMA panels) x 64 columns, or x 32 for the transposed product
-    (whose F32 gradient only has the 32-column variant at 32 rows).  Measured
-    faster than the per-call tuned choices on the ten-product suite and the
-    twelve-layer forward + backward step."""
+    serves two MFMA panels) x 64 columns, forward and transposed (a batch
+    reads an F32 gradient through its own F16 copy, which has the 64-column
+    variant).  Measured faster than the per-call tuned choices on the
+    ten-product suite and the twelve-layer forward + backward step (there the
+    transposed products at 32 x 64 took the step from 193 to 164 us; 32 x 32
+    had been the best while the F32 gradient was gathered directly)."""
     if batch:
         t64 = 64 if N >= 64 else (32 if N >= 32 else 16)
         if dense and N >= 32:
-            return 32, (32 if op == "spmm_t" else t64)
+            return 32, t64
         return 16, t64
     d = nnz / float(max(1, M * K))
     layout = "dense" if dense else "tiled"

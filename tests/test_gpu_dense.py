@@ -197,10 +197,13 @@ def test_dense_wide_plans(cuda, rpw):
         assert rel_l2(got, T.spmm(tr, w16, dev(G), tile_n=t).cpu().numpy()) <= 1e-6
 
 
-@pytest.mark.parametrize("rpw,tile,g_f32", [(32, 32, False), (32, 64, False), (64, 32, False), (32, 32, True)])
+@pytest.mark.parametrize("rpw,tile,g_f32", [(32, 32, False), (32, 64, False), (64, 32, False), (32, 32, True),
+                                            (32, 64, True), (64, 32, True)])
 def test_dense_batch_wider_plans(cuda, rpw, tile, g_f32):
     """Dense batch variants on 32- and 64-row plans (forward with an F16 B,
-    transposed with an F16 or F32 gradient) against per-call dense launches."""
+    transposed with an F16 or F32 gradient -- the batch reads an F32 one
+    through its F16 copy, so every F16 shape serves it) against per-call dense
+    launches."""
     keep, items, refs = [], [], []
     for case in list(workloads.batch12(seed=0))[:4] + list(workloads.batch12(seed=0))[8:10]:
         W, X, G = case.A, case.B, case.extra["G"]
