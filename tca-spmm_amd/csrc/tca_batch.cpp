@@ -68,6 +68,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             else if (f->rows != o.rows || f->N != o.N || f->ld != o.ld) f->ok = false;
         }
     ops.erase(std::remove_if(ops.begin(), ops.end(), [](const Operand &x) { return !x.ok; }), ops.end());
+    if (ops.size() > (size_t)tca::kMaxConv) ops.resize(tca::kMaxConv);   // the rest stay FP32
     const size_t conv_bytes = (sizeof(tca::ConvDesc) * std::max<size_t>(ops.size(), 1) + 255) / 256 * 256;
     size_t shadow_bytes = conv_bytes;
     for (Operand &o : ops) {

@@ -149,7 +149,9 @@ struct ConvDesc {              // an FP32 operand a batch rounds to an FP16 copy
     int64_t first8;            // its first 8-element group in the conversion launch (a multiple of 256)
 };
 // Rounds every operand of descs (nearest even, the rounding the kernels apply
-// on load) into its FP16 copy: one launch over total8 groups of 8 elements.
+// on load) into its FP16 copy: one launch over total8 groups of 8 elements,
+// at most kMaxConv operands.
+constexpr int kMaxConv = 256;
 tca_status launch_convert_f16(const ConvDesc *descs, int n, long total8, hipStream_t stream);
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // Work counters of a batch launch: the unit list is cut into up to

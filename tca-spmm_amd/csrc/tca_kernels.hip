@@ -1682,14 +1682,17 @@ namespace {
 
 // One thread per 8 consecutive elements of one row: two 16-byte loads, one
 // 16-byte store.  Every operand starts on a 256-group boundary (first8), so a
-// workgroup's operand is uniform: found by a scalar scan of the (few)
-// descriptors.
+// workgroup's operand is uniform; the operands' first groups are staged in LDS
+// once (one load per thread, at most kMaxConv operands) and scanned there.
 __global__ __launch_bounds__(256) void convert_f16_kernel(const ConvDesc *descs, int n, long total8)
 {
+    __shared__ long first[kMaxConv];
+    if ((int)threadIdx.x < n) first[threadIdx.x] = gbl(descs)[threadIdx.x].first8;
+    __syncthreads();
     for (long blk = blockIdx.x; blk * 256 < total8; blk += gridDim.x) {
         const long t0 = blk * 256;
         int k = 0;
-        while (k + 1 < n && descs[k + 1].first8 <= t0) k++;
+        while (k + 1 < n && first[k + 1] <= t0) k++;
         const float *src0 = descs[k].src;
         uint16_t *dst0 = descs[k].dst;
         const int N = descs[k].N, ld = descs[k].ld;

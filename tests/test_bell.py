@@ -128,3 +128,17 @@ def test_library_converter_rejections():
     bad[0] = 20
     with pytest.raises(_lib.TcaError):
         T.bell_from_csr(20, 20, rp, bad, None, 16)
+
+
+def test_blocked_ell_class_checks_shapes():
+    torch = pytest.importorskip("torch")
+    ci, v, ell_cols = T.bell_from_csr(40, 64, *gen.pattern("bernoulli", 40, 64, 0.2, seed=2, stream=2), None, 16)
+    vals = torch.from_numpy(v.view(np.int16)).view(torch.float16)
+    bell = T.BlockedEll(40, 64, 16, torch.from_numpy(ci), vals)
+    assert bell.ell_cols == ell_cols and tuple(bell.col_ind.shape) == (3, ell_cols // 16)
+    with pytest.raises(ValueError):
+        T.BlockedEll(40, 64, 16, torch.from_numpy(ci[:2]), vals)        # block rows
+    with pytest.raises(ValueError):
+        T.BlockedEll(40, 64, 16, torch.from_numpy(ci), vals[:32])       # padded rows
+    assert T.bell_shape(2048, 256, 32) == (16, 16)                       # narrowed to >= 2048 waves
+    assert T.bell_shape(8192, 1024, 32) == (32, 64)
