@@ -1625,10 +1625,12 @@ SddmmFn pick_sddmm(tca_dtype g_dtype, tca_dtype x_dtype)
 // SDDMM on the plan's 64-row tiling (sddmm_tiling), dW written in the plan's
 // tiled order.  Blocks of 4 chunks per workgroup, halved (to 2, then 1) while
 // that leaves fewer than kSddmmMinGroups workgroups: every chunk block re-reads
-// its group's G rows, so larger blocks win down to about one workgroup per CU
-// (measured on the twelve-layer workload: 1024 -> 256 took the twelve dW calls
-// from 148 to 117 us; blocks of 8 gained nothing more).
-static constexpr int kSddmmMinGroups = 256;
+// its group's G rows, so larger blocks win down to a little under one
+// workgroup per CU (measured on the twelve-layer workload: 1024 -> 256 took the
+// twelve dW calls from 148 to 117 us, 200 to 108 us -- the 1024 x 1024 layer at
+// N = 1024 then runs 240 blocks of 2 instead of 480 of 1 --, 128 was slower
+// again; blocks of 8 gained nothing).
+static constexpr int kSddmmMinGroups = 200;
 
 tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int ldg, tca_dtype x_dtype, const void *X,
                         int ldx, int N, float *out, hipStream_t stream)
