@@ -220,8 +220,9 @@ tca_status tca_bell_from_csr(int n_rows, int n_cols, const int32_t *row_offset, 
 tca_status tca_bell_spmm(int n_rows, int n_cols, int block, int ell_cols, const int32_t *d_col_ind,
                          tca_dtype a_dtype, const void *d_values, tca_dtype b_dtype, const void *d_B, int ldb,
                          int N, float *d_C, int ldc, int tile_n, int rows_per_wave, void *stream);
-/* The (rows_per_wave, tile_n) tca_bell_spmm picks for a shape when given 0s. */
-tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile);
+/* The (rows_per_wave, tile_n) tca_bell_spmm (and tca_spmm_24, block 32) picks
+ * for an n_rows x n_cols matrix times N columns when given 0s. */
+tca_status tca_bell_shape(int n_rows, int n_cols, int N, int block, int tile_n, int *rows_per_wave, int *tile);
 
 /* ------------------------------------------------------------------------ */
 /* 2:4 structured sparsity on gfx950's sparse MFMA.  Replaces                 */

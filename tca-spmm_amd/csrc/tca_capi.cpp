@@ -408,11 +408,12 @@ extern "C" tca_status tca_spmm_24(int n_rows, int n_cols, int N, tca_dtype a_dty
                               rows_per_wave, static_cast<hipStream_t>(stream));
 }
 
-extern "C" tca_status tca_bell_shape(int n_rows, int N, int block, int tile_n, int *rows_per_wave, int *tile)
+extern "C" tca_status tca_bell_shape(int n_rows, int n_cols, int N, int block, int tile_n, int *rows_per_wave,
+                                     int *tile)
 {
     if (!rows_per_wave || !tile || block < 16 || (block & (block - 1)))
         return fail(TCA_ERR_INVALID_ARG, "tca_bell_shape: bad argument");
-    tca::bell_shape(n_rows, N, block, tile_n, rows_per_wave, tile);
+    tca::bell_shape(n_rows, n_cols, N, block, tile_n, rows_per_wave, tile);
     return TCA_OK;
 }
 

@@ -110,7 +110,7 @@ tca_status sddmm_units(const tca_plan *aux, int cb, const int32_t **units, int *
 uint32_t operand_bytes(long rows, long N, long ld, int elem_bytes);
 
 // Blocked-ELL SpMM (tca_bell.hip): shape choice and launch.
-void bell_shape(int M, int N, int block, int tile_n, int *rpw, int *nt);
+void bell_shape(int M, int K, int N, int block, int tile_n, int *rpw, int *nt);
 // 2:4 sparse MFMA path (tca_sparse24.hip).
 tca_status launch_compress24(int M, int K, tca_dtype dtype, const void *dense, int ld, void *values, uint16_t *meta,
                              hipStream_t stream);

@@ -288,7 +288,7 @@ tca_status launch_spmm24(int M, int K, int N, tca_dtype a_dtype, const void *val
                          hipStream_t stream)
 {
     int rpw = 0, nt = 0;
-    bell_shape(M, N, 32, tile_n, &rpw, &nt);   // same unit-count rule as the Blocked-ELL kernel
+    bell_shape(M, K, N, 32, tile_n, &rpw, &nt);   // same unit-count rule as the Blocked-ELL kernel
     if (rows_per_wave) rpw = rows_per_wave;
     S24Fn fn = pick_s24(a_dtype, b_dtype, rpw, nt);
     if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_24: unsupported dtype pair or tile");

@@ -85,7 +85,7 @@ SIGNATURES = {
     "tca_sddmm": (_i, [_vp, _i, _vp, _i, _i, _vp, _i, _i, _vp, _vp]),
     "tca_bell_from_csr": (_i, [_i, _i, _i32p, _i32p, _u16p, _i, ctypes.POINTER(_i), _i32p, _u16p]),
     "tca_bell_spmm": (_i, [_i, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _i, _i, _i, _vp]),
-    "tca_bell_shape": (_i, [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "tca_bell_shape": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tca_compress_24": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp]),
     "tca_spmm_24": (_i, [_i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp]),
     "tca_quantize_rows_e4m3": (_i, [_i, _vp, _vp, _vp, _vp, _vp]),

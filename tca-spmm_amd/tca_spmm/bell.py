@@ -45,10 +45,10 @@ def bell_from_csr(n_rows, n_cols, row_offset, col_idx, values=None, block=32):
             ell_cols)
 
 
-def bell_shape(n_rows, N, block, tile_n=0):
-    """(rows_per_wave, tile_n) that bell_spmm picks for this shape."""
+def bell_shape(n_rows, n_cols, N, block, tile_n=0):
+    """(rows_per_wave, tile_n) that bell_spmm picks for an n_rows x n_cols matrix times N columns."""
     r, t = ctypes.c_int(0), ctypes.c_int(0)
-    call("tca_bell_shape", n_rows, N, block, tile_n, ctypes.byref(r), ctypes.byref(t))
+    call("tca_bell_shape", n_rows, n_cols, N, block, tile_n, ctypes.byref(r), ctypes.byref(t))
     return r.value, t.value
 
 

@@ -140,5 +140,7 @@ def test_blocked_ell_class_checks_shapes():
         T.BlockedEll(40, 64, 16, torch.from_numpy(ci[:2]), vals)        # block rows
     with pytest.raises(ValueError):
         T.BlockedEll(40, 64, 16, torch.from_numpy(ci), vals[:32])       # padded rows
-    assert T.bell_shape(2048, 256, 32) == (16, 16)                       # narrowed to >= 2048 waves
-    assert T.bell_shape(8192, 1024, 32) == (32, 64)
+    assert T.bell_shape(2048, 512, 256, 32) == (32, 32)                  # >= 512 waves for K = 512
+    assert T.bell_shape(512, 2048, 256, 32) == (16, 16)                  # >= 2048 waves for K = 2048
+    assert T.bell_shape(8192, 1024, 1024, 32) == (32, 64)
+    assert T.bell_shape(100, 64, 8, 16) == (16, 16)                      # rows per wave at most the block
