@@ -20,7 +20,9 @@ _EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 
 def hipsparselt_spmm_24(A_bits, B_bits, n_iter=20, timeout=120):
     """A (M x K) and B (K x N) as F16 bits -> (pruned A bits, C = pruned A @ B as F16
-    bits, mean ms per call over n_iter calls)."""
+    bits, mean ms per call over n_iter calls).  The library leg is switched off
+    for now (tca_hipsparselt_spmm_24 returns TCA_ERR_UNSUPPORTED, so this raises
+    RuntimeError); Sparse24 / spmm_24 are the 2:4 path."""
     A = np.ascontiguousarray(A_bits, np.uint16)
     B = np.ascontiguousarray(B_bits, np.uint16)
     (M, K), N = A.shape, B.shape[1]
