@@ -91,7 +91,7 @@ def time_eager(torch, fn, iters, backlog=False):
     return s.elapsed_time(e) / iters
 
 
-def bench_fwd_bwd(torch, np, T, O, dev):
+def bench_fwd_bwd(torch, np, T, O, dev, libs=True):
     """Twelve uneven layers, forward + dX + dW: per-call path (36 launches,
     tuned tiled-values launches and tuned panel-dense launches) vs one
     persistent launch (panel-dense forward and dX, SDDMM dW) replayed back to
@@ -106,7 +106,7 @@ def bench_fwd_bwd(torch, np, T, O, dev):
         w_vals = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
         Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
         Gd = torch.from_numpy(G).to(dev)
-        L = dict(X=Xd, G=Gd, w_vals=w_vals,
+        L = dict(X=Xd, G=Gd, w_vals=w_vals, row_offset=W.row_offset, col_idx=W.col_idx,
                  Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
                  dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
                  dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
@@ -178,12 +178,29 @@ def bench_fwd_bwd(torch, np, T, O, dev):
     repack_ms = time_graph(torch, repack, 20)
     per_op = {lay: {op: time_graph(torch, per_call(lay, op), 20) for op in ("fwd", "dx", "dw")} for lay in ("t", "d")}
     best_plain = min(plain_graph, dense_graph)
+    # library counterpart of the dW step: rocSPARSE SDDMM per layer (G rounded to FP16
+    # beforehand, the rounding the kernels apply on load)
+    lib_dw = None
+    if libs:
+        try:
+            lib_dw = 0.0
+            for L in layers:
+                W = L["sddmm_plan"]
+                ro = torch.from_numpy(L["row_offset"]).to(dev)
+                ci = torch.from_numpy(L["col_idx"]).to(dev)
+                out = torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)
+                lib_dw += T.rocsparse_sddmm(W.n_rows, W.n_cols, L["X"].shape[1], ro, ci, L["G"].half(), L["X"], out,
+                                            n_iter=10)
+        except Exception as ex:  # report, do not hide
+            lib_dw = float("nan")
+            log("rocSPARSE SDDMM failed:", ex)
     return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph,
             "dense_per_call_graph_ms": dense_graph, "fused_ms": fused, "fused_graph_ms": fused_graph,
             "speedup_vs_graph": best_plain / fused, "speedup_vs_tiled_graph": plain_graph / fused,
             "speedup_vs_eager": plain_eager / fused, "repack_ms": repack_ms,
             "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
+            "rocsparse_dw_ms": lib_dw,
             "launches_plain": 3 * len(layers), "units": bt.n_units, "workgroups": bt.n_workgroups}
 
 
@@ -506,7 +523,7 @@ def main():
         extra["speedup_vs_hipblaslt"] = lib_ms["hipblaslt"] / ms_per_step if lib_ms["hipblaslt"] else None
 
     if "fwdbwd" in args.sections:
-        extra["fwd_bwd_12_layers"] = bench_fwd_bwd(torch, np, T, O, dev)
+        extra["fwd_bwd_12_layers"] = bench_fwd_bwd(torch, np, T, O, dev, libs=not args.no_libs)
     if "precision" in args.sections:
         extra["precision_4096"] = bench_precision(torch, np, T, O, dev)
 

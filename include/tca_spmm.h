@@ -311,6 +311,12 @@ tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row
 /* hipBLASLt dense GEMM C = A_dense * B, f16 in, f32 out (row-major). */
 tca_status tca_hipblaslt_gemm(int M, int N, int K, const uint16_t *d_A, const uint16_t *d_B, float *d_C,
                               int n_iter, double *ms_per_iter);
+/* rocSPARSE SDDMM (the library counterpart of tca_sddmm): out (CSR order, f32)
+ * = (G X^T) on the pattern, G M x N and X K x N row-major f16, FP32 compute.
+ * No counterpart in the reference (it has no backward); ms_per_iter as above. */
+tca_status tca_rocsparse_sddmm(int M, int K, int N, int nnz, const int32_t *d_row_offset, const int32_t *d_col_idx,
+                               const uint16_t *d_G, const uint16_t *d_X, float *d_out, int n_iter,
+                               double *ms_per_iter);
 /* 2:4 structured sparsity on hipSPARSELt.  Replaces perform_cuSPARSELT_24sparsity
  * (src/spmm_cusparse.cu:2-179, decl src/spmm_cusparse.cuh:13): A (dense row-major
  * M x K, F16) pruned with the STRIP rule (per 1x4 strip of a row the pair of

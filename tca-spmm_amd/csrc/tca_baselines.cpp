@@ -155,3 +155,63 @@ done:
     if (h) hipblasLtDestroy(h);
     return st;
 }
+
+// rocSPARSE SDDMM as the library counterpart of tca_sddmm (dW): out (CSR order,
+// f32) = (G Xt) sampled on the pattern, G M x N and X K x N row-major f16 (X read
+// as the column-major N x K matrix Xt), FP32 compute.  The reference has no
+// backward; this is the baseline for the dW of its SpMM.
+extern "C" tca_status tca_rocsparse_sddmm(int M, int K, int N, int nnz, const int32_t *d_row_offset,
+                                          const int32_t *d_col_idx, const uint16_t *d_G, const uint16_t *d_X,
+                                          float *d_out, int n_iter, double *ms_per_iter)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || nnz < 0 || !d_row_offset || !d_G || !d_X || (nnz > 0 && (!d_col_idx || !d_out)))
+        return fail(TCA_ERR_INVALID_ARG, "tca_rocsparse_sddmm: bad argument");
+    tca_status st = TCA_OK;
+    rocsparse_handle h = nullptr;
+    rocsparse_spmat_descr C = nullptr;
+    rocsparse_dnmat_descr A = nullptr, B = nullptr;
+    void *buf = nullptr;
+    size_t bs = 0;
+    const float alpha = 1.0f, beta = 0.0f;
+    EventPair ev;
+    TCA_SPARSE_TRY(rocsparse_create_handle(&h));
+    TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&A, M, N, N, const_cast<uint16_t *>(d_G), rocsparse_datatype_f16_r,
+                                                rocsparse_order_row));
+    TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&B, N, K, N, const_cast<uint16_t *>(d_X), rocsparse_datatype_f16_r,
+                                                rocsparse_order_column));
+    TCA_SPARSE_TRY(rocsparse_create_csr_descr(&C, M, K, nnz, const_cast<int32_t *>(d_row_offset),
+                                              const_cast<int32_t *>(d_col_idx), d_out, rocsparse_indextype_i32,
+                                              rocsparse_indextype_i32, rocsparse_index_base_zero,
+                                              rocsparse_datatype_f32_r));
+    TCA_SPARSE_TRY(rocsparse_sddmm_buffer_size(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B,
+                                               &beta, C, rocsparse_datatype_f32_r, rocsparse_sddmm_alg_default, &bs));
+    if (bs && hipMalloc(&buf, bs) != hipSuccess) { st = fail(TCA_ERR_ALLOC, "rocsparse buffer"); goto done; }
+    TCA_SPARSE_TRY(rocsparse_sddmm_preprocess(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B,
+                                              &beta, C, rocsparse_datatype_f32_r, rocsparse_sddmm_alg_default, buf));
+    {
+        const int warm = n_iter > 0 ? 10 : 1;
+        for (int i = 0; i < warm; i++)
+            TCA_SPARSE_TRY(rocsparse_sddmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta,
+                                           C, rocsparse_datatype_f32_r, rocsparse_sddmm_alg_default, buf));
+        if (n_iter > 0) {
+            if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) { st = fail(TCA_ERR_HIP, "event"); goto done; }
+            (void)hipEventRecord(ev.a, nullptr);
+            for (int i = 0; i < n_iter; i++)
+                TCA_SPARSE_TRY(rocsparse_sddmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B,
+                                               &beta, C, rocsparse_datatype_f32_r, rocsparse_sddmm_alg_default, buf));
+            (void)hipEventRecord(ev.b, nullptr);
+            (void)hipEventSynchronize(ev.b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev.a, ev.b);
+            if (ms_per_iter) *ms_per_iter = ms / n_iter;
+        }
+        if (hipDeviceSynchronize() != hipSuccess) st = fail(TCA_ERR_HIP, "rocsparse: synchronize");
+    }
+done:
+    if (buf) (void)hipFree(buf);
+    if (C) rocsparse_destroy_spmat_descr(C);
+    if (A) rocsparse_destroy_dnmat_descr(A);
+    if (B) rocsparse_destroy_dnmat_descr(B);
+    if (h) rocsparse_destroy_handle(h);
+    return st;
+}

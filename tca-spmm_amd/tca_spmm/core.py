@@ -271,3 +271,12 @@ def hipblaslt_gemm(M, N, K, A_dense, B, out, n_iter=0):
     call("tca_hipblaslt_gemm", M, N, K, _ptr(A_dense), _ptr(B), _ptr(out), n_iter, ctypes.byref(ms))
     return ms.value
 
+
+
+def rocsparse_sddmm(M, K, N, row_offset, col_idx, G16, X16, out, n_iter=0):
+    """rocSPARSE SDDMM baseline: out (CSR order, float32) = (G X^T) on the pattern,
+    G (M x N) and X (K x N) float16; returns mean ms per call (n_iter > 0)."""
+    ms = ctypes.c_double(0.0)
+    call("tca_rocsparse_sddmm", M, K, N, col_idx.numel(), _ptr(row_offset), _ptr(col_idx), _ptr(G16), _ptr(X16),
+         _ptr(out), n_iter, ctypes.byref(ms))
+    return ms.value
