@@ -180,8 +180,20 @@ def bench_fwd_bwd(torch, np, T, O, dev, libs=True):
     best_plain = min(plain_graph, dense_graph)
     # library counterpart of the dW step: rocSPARSE SDDMM per layer (G rounded to FP16
     # beforehand, the rounding the kernels apply on load)
-    lib_dw = None
+    lib_dw = lib_dx = None
     if libs:
+        try:
+            lib_dx = 0.0
+            for L in layers:
+                W = L["sddmm_plan"]
+                ro = torch.from_numpy(L["row_offset"]).to(dev)
+                ci = torch.from_numpy(L["col_idx"]).to(dev)
+                out = torch.empty((W.n_cols, L["X"].shape[1]), dtype=torch.float32, device=dev)
+                lib_dx += T.rocsparse_spmm_t(W.n_rows, L["X"].shape[1], W.n_cols, ro, ci, L["w_vals"], L["G"].half(), out,
+                                             n_iter=10)
+        except Exception as ex:  # report, do not hide
+            lib_dx = float("nan")
+            log("rocSPARSE transposed SpMM failed:", ex)
         try:
             lib_dw = 0.0
             for L in layers:
@@ -200,7 +212,7 @@ def bench_fwd_bwd(torch, np, T, O, dev, libs=True):
             "speedup_vs_eager": plain_eager / fused, "repack_ms": repack_ms,
             "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
-            "rocsparse_dw_ms": lib_dw,
+            "rocsparse_dw_ms": lib_dw, "rocsparse_dx_ms": lib_dx,
             "launches_plain": 3 * len(layers), "units": bt.n_units, "workgroups": bt.n_workgroups}
 
 

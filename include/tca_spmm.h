@@ -308,6 +308,11 @@ tca_status tca_batch_get_info(const tca_batch *batch, int *n_units, int *n_workg
 tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row_offset, const int32_t *d_col_idx,
                               const uint16_t *d_values, const uint16_t *d_B, float *d_C, int n_iter,
                               double *ms_per_iter);
+/* The transposed product C (K x N) = A^T B, B M x N: the library counterpart of
+ * the dX launch on a transposed plan (no counterpart in the reference). */
+tca_status tca_rocsparse_spmm_t(int M, int N, int K, int nnz, const int32_t *d_row_offset, const int32_t *d_col_idx,
+                                const uint16_t *d_values, const uint16_t *d_B, float *d_C, int n_iter,
+                                double *ms_per_iter);
 /* hipBLASLt dense GEMM C = A_dense * B, f16 in, f32 out (row-major). */
 tca_status tca_hipblaslt_gemm(int M, int N, int K, const uint16_t *d_A, const uint16_t *d_B, float *d_C,
                               int n_iter, double *ms_per_iter);

@@ -31,9 +31,9 @@ struct EventPair {
         }                                                                                             \
     } while (0)
 
-extern "C" tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row_offset,
-                                         const int32_t *d_col_idx, const uint16_t *d_values, const uint16_t *d_B,
-                                         float *d_C, int n_iter, double *ms_per_iter)
+static tca_status rocsparse_spmm_op(rocsparse_operation op, int M, int N, int K, int nnz, const int32_t *d_row_offset,
+                                    const int32_t *d_col_idx, const uint16_t *d_values, const uint16_t *d_B, float *d_C,
+                                    int n_iter, double *ms_per_iter)
 {
     if (M <= 0 || N <= 0 || K <= 0 || nnz < 0 || !d_row_offset || !d_B || !d_C)
         return fail(TCA_ERR_INVALID_ARG, "tca_rocsparse_spmm: bad argument");
@@ -50,27 +50,30 @@ extern "C" tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int
                                               const_cast<int32_t *>(d_col_idx), const_cast<uint16_t *>(d_values),
                                               rocsparse_indextype_i32, rocsparse_indextype_i32,
                                               rocsparse_index_base_zero, rocsparse_datatype_f16_r));
-    TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&B, K, N, N, const_cast<uint16_t *>(d_B), rocsparse_datatype_f16_r,
-                                                rocsparse_order_row));
-    TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&C, M, N, N, d_C, rocsparse_datatype_f32_r, rocsparse_order_row));
-    TCA_SPARSE_TRY(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta, C,
+    {
+        const int rb = op == rocsparse_operation_none ? K : M, rc = op == rocsparse_operation_none ? M : K;
+        TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&B, rb, N, N, const_cast<uint16_t *>(d_B), rocsparse_datatype_f16_r,
+                                                    rocsparse_order_row));
+        TCA_SPARSE_TRY(rocsparse_create_dnmat_descr(&C, rc, N, N, d_C, rocsparse_datatype_f32_r, rocsparse_order_row));
+    }
+    TCA_SPARSE_TRY(rocsparse_spmm(h, op, rocsparse_operation_none, &alpha, A, B, &beta, C,
                                   rocsparse_datatype_f32_r, rocsparse_spmm_alg_default,
                                   rocsparse_spmm_stage_buffer_size, &bs, nullptr));
     if (bs && hipMalloc(&buf, bs) != hipSuccess) { st = fail(TCA_ERR_ALLOC, "rocsparse buffer"); goto done; }
-    TCA_SPARSE_TRY(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta, C,
+    TCA_SPARSE_TRY(rocsparse_spmm(h, op, rocsparse_operation_none, &alpha, A, B, &beta, C,
                                   rocsparse_datatype_f32_r, rocsparse_spmm_alg_default,
                                   rocsparse_spmm_stage_preprocess, &bs, buf));
     {
         const int warm = n_iter > 0 ? 10 : 1;  // the reference warms cuSPARSE up 10x
         for (int i = 0; i < warm; i++)
-            TCA_SPARSE_TRY(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B, &beta,
+            TCA_SPARSE_TRY(rocsparse_spmm(h, op, rocsparse_operation_none, &alpha, A, B, &beta,
                                           C, rocsparse_datatype_f32_r, rocsparse_spmm_alg_default,
                                           rocsparse_spmm_stage_compute, &bs, buf));
         if (n_iter > 0) {
             if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) { st = fail(TCA_ERR_HIP, "event"); goto done; }
             (void)hipEventRecord(ev.a, nullptr);
             for (int i = 0; i < n_iter; i++)
-                TCA_SPARSE_TRY(rocsparse_spmm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, A, B,
+                TCA_SPARSE_TRY(rocsparse_spmm(h, op, rocsparse_operation_none, &alpha, A, B,
                                               &beta, C, rocsparse_datatype_f32_r, rocsparse_spmm_alg_default,
                                               rocsparse_spmm_stage_compute, &bs, buf));
             (void)hipEventRecord(ev.b, nullptr);
@@ -88,6 +91,24 @@ done:
     if (C) rocsparse_destroy_dnmat_descr(C);
     if (h) rocsparse_destroy_handle(h);
     return st;
+}
+
+extern "C" tca_status tca_rocsparse_spmm(int M, int N, int K, int nnz, const int32_t *d_row_offset,
+                                         const int32_t *d_col_idx, const uint16_t *d_values, const uint16_t *d_B,
+                                         float *d_C, int n_iter, double *ms_per_iter)
+{
+    return rocsparse_spmm_op(rocsparse_operation_none, M, N, K, nnz, d_row_offset, d_col_idx, d_values, d_B, d_C,
+                             n_iter, ms_per_iter);
+}
+
+// The transposed product C (K x N) = A^T B (B: M x N), A (M x K) in CSR: the
+// library counterpart of the dX launch on a transposed plan.
+extern "C" tca_status tca_rocsparse_spmm_t(int M, int N, int K, int nnz, const int32_t *d_row_offset,
+                                           const int32_t *d_col_idx, const uint16_t *d_values, const uint16_t *d_B,
+                                           float *d_C, int n_iter, double *ms_per_iter)
+{
+    return rocsparse_spmm_op(rocsparse_operation_transpose, M, N, K, nnz, d_row_offset, d_col_idx, d_values, d_B, d_C,
+                             n_iter, ms_per_iter);
 }
 
 #define TCA_LT_TRY(expr)                                                                              \

@@ -12,7 +12,7 @@ from .sparse24 import hipsparselt_spmm_24  # noqa: F401
 lib()  # fail loudly at import when the native library is absent
 
 try:  # torch-facing API (torch is the device-memory layer, not the compute)
-    from .core import (Plan, PlanCache, csr_transpose_device, hipblaslt_gemm, rocsparse_sddmm,  # noqa: F401
+    from .core import (Plan, PlanCache, csr_transpose_device, hipblaslt_gemm, rocsparse_sddmm, rocsparse_spmm_t,  # noqa: F401
                        quantize_cols_e4m3, quantize_rows_e4m3, rocsparse_spmm, sddmm, spmm)
     from .autograd import SparseLinear, SpmmFunction, sparse_matmul  # noqa: F401
     from .batch import Batch, sddmm_item, spmm_item  # noqa: F401
@@ -22,4 +22,4 @@ except ImportError:  # pragma: no cover - torch absent
     pass
 
 __all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
-           "hipblaslt_gemm", "rocsparse_sddmm", "hipsparselt_spmm_24", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "Sparse24", "spmm_24", "TcaError", "F16", "BF16", "E4M3", "F32"]
+           "hipblaslt_gemm", "rocsparse_sddmm", "rocsparse_spmm_t", "hipsparselt_spmm_24", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "Sparse24", "spmm_24", "TcaError", "F16", "BF16", "E4M3", "F32"]

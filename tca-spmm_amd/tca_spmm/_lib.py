@@ -97,6 +97,7 @@ SIGNATURES = {
     "tca_rocsparse_spmm": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _dp]),
     "tca_hipblaslt_gemm": (_i, [_i, _i, _i, _vp, _vp, _vp, _i, _dp]),
     "tca_rocsparse_sddmm": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _dp]),
+    "tca_rocsparse_spmm_t": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _dp]),
     "tca_hipsparselt_spmm_24": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _i, _dp]),
 }
 

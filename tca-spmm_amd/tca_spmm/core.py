@@ -265,6 +265,15 @@ def rocsparse_spmm(M, N, K, row_offset, col_idx, values, B, out, n_iter=0):
     return ms.value
 
 
+def rocsparse_spmm_t(M, N, K, row_offset, col_idx, values, B, out, n_iter=0):
+    """rocSPARSE transposed CSR SpMM baseline: out (K x N, f32) = A^T B, B (M x N) f16;
+    returns mean ms per call (n_iter > 0)."""
+    ms = ctypes.c_double(0.0)
+    call("tca_rocsparse_spmm_t", M, N, K, col_idx.numel(), _ptr(row_offset), _ptr(col_idx), _ptr(values), _ptr(B),
+         _ptr(out), n_iter, ctypes.byref(ms))
+    return ms.value
+
+
 def hipblaslt_gemm(M, N, K, A_dense, B, out, n_iter=0):
     """hipBLASLt dense GEMM baseline (f16 -> f32); returns mean ms per call (n_iter > 0)."""
     ms = ctypes.c_double(0.0)

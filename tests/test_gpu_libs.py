@@ -51,3 +51,14 @@ def test_rocsparse_sddmm_matches_oracle_and_tca_sddmm():
     plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, 16)
     ours = plan.unpack(T.sddmm(plan, dev_f16(G), dev_f16(X)))
     assert rel_l2(ours.cpu().numpy(), ref) <= 1e-5
+
+
+def test_rocsparse_transposed_spmm_matches_oracle():
+    A, _ = _case()                               # A (300 x 200): C (200 x 64) = A^T G
+    G = gen.f32_to_f16_bits(gen.normal_f32(3, 4, (A.n_rows, 64)))
+    C = torch.zeros((A.n_cols, 64), dtype=torch.float32, device="cuda")
+    T.rocsparse_spmm_t(A.n_rows, 64, A.n_cols, dev(A.row_offset), dev(A.col_idx), dev_f16(A.values), dev_f16(G), C)
+    torch.cuda.synchronize()
+    rpT, cT, perm = O.transpose_csr(A.n_rows, A.n_cols, A.row_offset, A.col_idx)
+    ref = O.spmm_f32(A.n_cols, 64, rpT, cT, A.values[perm], G)
+    assert rel_l2(C.cpu().numpy(), ref) <= 1e-5
