@@ -309,7 +309,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md and is not run.", file=sys.stderr)
+              "the path would shard by row groups (B replicated, dX reduce-scattered), which is not run here.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -2,8 +2,8 @@
 
 Bars: FP16/BF16/FP8 inputs with FP32 accumulation must match the oracle run on
 the same (quantised) inputs to rel-L2 <= 1e-5 (accumulation-order noise only),
-and the FP64 oracle on the original values to rel-L2 <= 1e-3 (north-star
-tolerance) for FP16; the reference's compare_mat rule (tol 1e-3) must pass.
+and the FP64 oracle on the original values to rel-L2 <= 1e-3 (the path's
+stated tolerance) for FP16; the reference's compare_mat rule (tol 1e-3) must pass.
 Exact cases (identity, single nonzero per row, empty rows) are bit-exact.
 """
 import os
