@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the row-balanced CSR x dense FP16 SpMM on one MI355X.
+"""Benchmark of the row-balanced CSR x dense SpMM path on one MI355X.
 
-Workload (one "step"): the FFN-shaped GPU-forward suite -- CSR A of 2048x512
-and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 (lognormal(0.5) row lengths,
-values N(0,0.02)), B = K x 256 N(0,1), FP16 inputs, FP32 accumulate and output:
-ten SpMMs per step, inputs resident in HBM (A as panel-dense 16 x 32 tiles of
-32-row plans, built once like the tiled values), run as ONE persistent batched
-launch (the ten products' work units drain through one grid), K launches back
-to back.  value = sum(2*nnz*N) / step time (TFLOP/s).
+Headline workload (one "step"): the twelve-layer sparse-weight training step --
+twelve uneven CSR layers (4x 512x2048, 4x 2048x512, 2x 1024x1024,
+2x 4096x1024, sparsity U[0.5, 0.98], lognormal(0.5) rows, N in
+{128, 384, 1024}), FP16 weights and activations, FP32 upstream gradient G.
+A step is: the SGD update of every weight from the previous step's dW
+(w -= lr * dW, one elementwise launch over all layers' values), then forward
+Y = W X, dX = W^T G and dW = SDDMM(G, X^T) of all twelve layers.  The fused
+path runs it as the batch's prologue launch (FP16 copy of every G, panel-dense
+tiles of every W rebuilt from the updated values) + ONE persistent launch over
+all forward / dX / dW work units.  value = sum over layers of 3 * 2*nnz*N
+(forward, dX, dW) / step time, in TFLOP/s.
 
-Also reported: the HBM roofline of that launch (the ten products' algorithmic
-bytes / its measured duration vs 8 TB/s), the ten tuned per-call launches
-(dense tiles: graph and eager; tiled values: graph), per-matrix warm and cold
-launch times, rocSPARSE and hipBLASLt on the same inputs, and the CPU oracle
-(FP32-accumulate C port of naive_gemm restricted to nonzeros) timed on the
-host cores.
-Prints one JSON line on stdout.
+Beside it, in the same run: the plain per-call FP16 path (36 tuned launches of
+tca_spmm / tca_sddmm on tiled values, with its own repack and the same SGD
+update, one HIP graph) and the per-call panel-dense path with its repack; the
+roofline of the persistent launch (the step's algorithmic bytes / its measured
+duration vs 8 TB/s); the CPU oracle on the same step (all host cores); the
+plain GPU suite (ten FFN-shaped products) as one batched launch, per call,
+Blocked-ELL, 2:4, rocSPARSE and hipBLASLt; the 4096^2 x 512 precision sweep
+with hipBLASLt on the densified matrix; the 512x512 CPU-runnable case on the
+GPU and on the CPU.  Prints one JSON line on stdout.
 """
 import argparse
 import json
@@ -29,7 +35,9 @@ for _p in (ROOT, os.path.join(ROOT, "tca-spmm_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0
 METRIC = "SpMM eff. TFLOP/s (2·nnz·N) + %HBM roofline; fwd+bwd ms vs plain FP16"
+SGD_LR = 1e-6
 
 
 def log(*a):
@@ -91,145 +99,386 @@ def time_eager(torch, fn, iters, backlog=False):
     return s.elapsed_time(e) / iters
 
 
-def bench_fwd_bwd(torch, np, T, O, dev, libs=True):
-    """Twelve uneven layers, forward + dX + dW: per-call path (36 launches,
-    tuned tiled-values launches and tuned panel-dense launches) vs one
-    persistent launch (panel-dense forward and dX, SDDMM dW) replayed back to
-    back and from a HIP graph.  Weights are fixed over the timed steps, so
-    their tiled values and dense tiles are built before timing; `repack_ms`
-    is what rebuilding them after a weight update costs per step."""
-    from tca_spmm import workloads
-    from tca_spmm.workloads import alg_bytes_dw, alg_bytes_dx, alg_bytes_fwd
-    layers, items, flops, nbytes = [], [], 0, 0
-    for case in workloads.batch12(seed=0):
-        W, X, G = case.A, case.B, case.extra["G"]
-        w_vals = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
-        Xd = torch.from_numpy(X.view(np.int16)).view(torch.float16).to(dev)
-        Gd = torch.from_numpy(G).to(dev)
-        L = dict(X=Xd, G=Gd, w_vals=w_vals, row_offset=W.row_offset, col_idx=W.col_idx,
-                 Y=torch.empty((W.n_rows, case.N), dtype=torch.float32, device=dev),
-                 dX=torch.empty((W.n_cols, case.N), dtype=torch.float32, device=dev),
-                 dW=torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)[:W.nnz])
-        fwd16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
-        L["w16"] = fwd16.pack(w_vals)
-        L["sddmm_plan"] = fwd16
-        # per-call paths: the tuned launch of each product, tiled values and dense tiles
-        for lay, dense in (("t", False), ("d", True)):
-            rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, dense=dense)
-            rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", dense=dense)
-            plan = fwd16 if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-            w = L["w16"] if rpw == 16 else plan.pack(w_vals)
-            plan_t = fwd16.transpose(rpw_t)
-            L[lay] = dict(plan=plan, plan_t=plan_t, tile=tile, tile_t=tile_t, dense=dense,
-                          a=plan.pack_dense(w) if dense else w,
-                          at=plan_t.pack_dense(L["w16"]) if dense else L["w16"])
-        # batched path: panel-dense forward and dX on the batch's choices, SDDMM dW
-        rpw_b, tile_b = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, case.N, batch=True, dense=True)
-        rpw_bt, tile_bt = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, case.N, op="spmm_t", batch=True, dense=True)
-        pb = fwd16 if rpw_b == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw_b)
-        wb = L["w16"] if rpw_b == 16 else pb.pack(w_vals)
-        pbt = fwd16.transpose(rpw_bt)
-        L["b"] = dict(plan=pb, plan_t=pbt, wb=wb, a=pb.pack_dense(wb), at=pbt.pack_dense(L["w16"]))
-        items += [T.spmm_item(pb, L["b"]["a"], Xd, L["Y"], tile_b, dense=True),
-                  T.spmm_item(pbt, L["b"]["at"], Gd, L["dX"], tile_bt, dense=True),
-                  T.sddmm_item(fwd16, Gd, Xd, L["dW"])]
-        flops += 3 * 2 * W.nnz * case.N
-        nbytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, case.N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, case.N)
-                   + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, case.N))
-        layers.append(L)
+def rel_l2(np, got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
 
-    def per_call(lay, op=None):
+
+# --------------------------------------------------------------------------
+# the twelve-layer training step
+# --------------------------------------------------------------------------
+class TrainStep:
+    """Weights of all layers in one FP16 buffer (each in its 16-row plan's
+    tiled order, the order SDDMM writes dW in), dW of all layers in one FP32
+    buffer, so the SGD update is one launch.  Three ways to run the step."""
+
+    def __init__(self, torch, np, T, dev, seed=0):
+        from tca_spmm import workloads
+        from tca_spmm.workloads import alg_bytes_dw, alg_bytes_dx, alg_bytes_fwd
+        self.torch, self.np, self.T = torch, np, T
+        cases = workloads.batch12(seed=seed)
+        offs, tot = [], 0
+        for c in cases:
+            offs.append(tot)
+            tot += (c.A.nnz + 7) // 8 * 8            # 16-byte aligned views
+        self.w_all = torch.zeros(max(tot, 8), dtype=torch.float16, device=dev)
+        self.dw_all = torch.zeros(max(tot, 8), dtype=torch.float32, device=dev)
+        self.layers, self.flops, self.alg_bytes, self.dense_flops = [], 0, 0, 0
+        items = []
+        for c, off in zip(cases, offs):
+            W, N = c.A, c.N
+            L = dict(case=c, N=N)
+            p16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+            v_csr = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
+            w = self.w_all[off:off + W.nnz]
+            w.copy_(p16.pack(v_csr))
+            L.update(p16=p16, w=w, dW=self.dw_all[off:off + W.nnz],
+                     X=torch.from_numpy(c.B.view(np.int16)).view(torch.float16).to(dev),
+                     G=torch.from_numpy(c.extra["G"]).to(dev),
+                     Y=torch.empty((W.n_rows, N), dtype=torch.float32, device=dev),
+                     dX=torch.empty((W.n_cols, N), dtype=torch.float32, device=dev))
+            # fused: the batch packs dense tiles of the forward and transposed plans
+            rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)
+            rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True)
+            pf = p16 if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+            pt = p16.transpose(rpw_t)
+            L["fused"] = dict(fwd=(rpw, tile), dx=(rpw_t, tile_t))
+            items += [T.spmm_item(pf, w, L["X"], L["Y"], tile, dense="pack", values_order=p16),
+                      T.spmm_item(pt, w, L["G"], L["dX"], tile_t, dense="pack", values_order=p16),
+                      T.sddmm_item(p16, L["G"], L["X"], L["dW"])]
+            # per call: each product's tuned launch, tiled values ("t") and dense tiles ("d")
+            for lay, dense in (("t", False), ("d", True)):
+                r, t = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, dense=dense)
+                r_t, t_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", dense=dense)
+                plan = p16 if r == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, r)
+                plan_t = p16.transpose(r_t)
+                own = None if plan is p16 else torch.empty_like(w)          # plan's own tiled values
+                L[lay] = dict(plan=plan, plan_t=plan_t, tile=t, tile_t=t_t, own=own, choice=[[r, t], [r_t, t_t]],
+                              a=(plan.pack_dense(w if own is None else own) if dense else None),
+                              at=(plan_t.pack_dense(w) if dense else None))
+            L["csr"] = torch.empty_like(w)
+            self.layers.append(L)
+            self.flops += 3 * 2 * W.nnz * N
+            self.dense_flops += 3 * 2 * W.n_rows * W.n_cols * N
+            self.alg_bytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, N)
+                               + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, N))
+        self.batch = T.Batch(items)
+
+    def sgd(self):
+        self.w_all.add_(self.dw_all, alpha=-SGD_LR)
+
+    def repack(self, lay):
+        """What the per-call path `lay` rebuilds after a weight update: its own
+        tiled values of plans other than the 16-row one, and its dense tiles."""
+        for L in self.layers:
+            c = L[lay]
+            if c["own"] is not None:
+                L["p16"].unpack(L["w"], out=L["csr"])
+                c["plan"].pack(L["csr"], out=c["own"])
+            if lay == "d":
+                c["plan"].pack_dense(L["w"] if c["own"] is None else c["own"], out=c["a"])
+                c["plan_t"].pack_dense(L["w"], out=c["at"])
+
+    def products(self, lay, op=None):
+        T = self.T
+        for L in self.layers:
+            c = L[lay]
+            dense = lay == "d"
+            a = c["a"] if dense else (L["w"] if c["own"] is None else c["own"])
+            if op in (None, "fwd"):
+                T.spmm(c["plan"], a, L["X"], out=L["Y"], tile_n=c["tile"], dense=dense)
+            if op in (None, "dx"):
+                T.spmm(c["plan_t"], c["at"] if dense else L["w"], L["G"], out=L["dX"], tile_n=c["tile_t"], dense=dense)
+            if op in (None, "dw"):
+                T.sddmm(L["p16"], L["G"], L["X"], out=L["dW"])
+
+    def fused_step(self):
+        self.sgd()
+        self.batch.run()
+
+    def per_call_step(self, lay):
         def run():
-            for L in layers:
-                c = L[lay]
-                if op in (None, "fwd"):
-                    T.spmm(c["plan"], c["a"], L["X"], out=L["Y"], tile_n=c["tile"], dense=c["dense"])
-                if op in (None, "dx"):
-                    T.spmm(c["plan_t"], c["at"], L["G"], out=L["dX"], tile_n=c["tile_t"], dense=c["dense"])
-                if op in (None, "dw"):
-                    T.sddmm(L["sddmm_plan"], L["G"], L["X"], out=L["dW"])
+            self.sgd()
+            self.repack(lay)
+            self.products(lay)
         return run
 
-    def repack():
-        for L in layers:
-            L["sddmm_plan"].pack(L["w_vals"], out=L["w16"])
-            if L["b"]["wb"] is not L["w16"]:
-                L["b"]["plan"].pack(L["w_vals"], out=L["b"]["wb"])
-            L["b"]["plan"].pack_dense(L["b"]["wb"], out=L["b"]["a"])
-            L["b"]["plan_t"].pack_dense(L["w16"], out=L["b"]["at"])
+    def parity_gate(self, O):
+        """Fused outputs of every layer against the per-call tiled launches
+        (FP32 summation order) and two layers against the FP64 oracle."""
+        torch, np = self.torch, self.np
+        self.batch.run()
+        torch.cuda.synchronize()
+        fused = [(L["Y"].clone(), L["dX"].clone(), L["dW"].clone()) for L in self.layers]
+        self.repack("t")
+        self.products("t")
+        torch.cuda.synchronize()
+        worst = 0.0
+        for L, (y, dx, dw) in zip(self.layers, fused):
+            for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["dW"])):
+                r = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
+                worst = max(worst, r)
+                if r > 1e-6:
+                    raise RuntimeError(f"{L['case'].A.name}: fused step differs from the per-call launches ({r:.2e})")
+        oracle_err = {}
+        for i in (0, len(self.layers) - 1):
+            L = self.layers[i]
+            W, N = L["case"].A, L["N"]
+            a64 = L["p16"].unpack(L["w"]).cpu().numpy().view(np.float16).astype(np.float64)
+            x64 = L["case"].B.view(np.float16).astype(np.float64)
+            g64 = L["case"].extra["G"].astype(np.float16).astype(np.float64)
+            ref_y = O.spmm_f64(W.n_rows, N, W.row_offset, W.col_idx, a64, x64)
+            rpT, cT, perm = O.transpose_csr(W.n_rows, W.n_cols, W.row_offset, W.col_idx)
+            ref_dx = O.spmm_f64(W.n_cols, N, rpT, cT, a64[perm], g64)
+            ref_dw = O.sddmm_f64(W.n_rows, N, W.row_offset, W.col_idx, g64, x64)
+            y, dx, dw = fused[i]
+            e = [rel_l2(np, y.cpu().numpy(), ref_y), rel_l2(np, dx.cpu().numpy(), ref_dx),
+                 rel_l2(np, L["p16"].unpack(dw).cpu().numpy(), ref_dw)]
+            if max(e) > 1e-5:
+                raise RuntimeError(f"{W.name}: fused step vs FP64 oracle {e}")
+            oracle_err[W.name] = {"fwd": e[0], "dx": e[1], "dw": e[2]}
+        return worst, oracle_err
 
-    bt = T.Batch(items)
-    # parity gate: the fused launch against the per-call dense-tile launches
-    bt.run()
+    def cpu_step(self, O, threads):
+        """The same step's products on the CPU oracle (FP32 accumulation of FP16
+        products): forward, dX on the transposed CSR, dW, every layer."""
+        np = self.np
+        for L in self.layers:
+            W, N = L["case"].A, L["N"]
+            if "cpu" not in L:
+                rpT, cT, perm = O.transpose_csr(W.n_rows, W.n_cols, W.row_offset, W.col_idx)
+                g16 = L["case"].extra["G"].astype(np.float16).view(np.uint16)
+                L["cpu"] = (rpT, cT, np.ascontiguousarray(W.values[perm]), g16)
+            rpT, cT, vT, g16 = L["cpu"]
+            O.spmm_f32(W.n_rows, N, W.row_offset, W.col_idx, W.values, L["case"].B, threads)
+            O.spmm_f32(W.n_cols, N, rpT, cT, vT, g16, threads)
+            O.sddmm_f32(W.n_rows, N, W.row_offset, W.col_idx, g16, L["case"].B, threads)
+
+
+def bench_train_step(torch, np, T, O, dev, args):
+    ts = TrainStep(torch, np, T, dev)
+    worst, oracle_err = ts.parity_gate(O)
+    log(f"train step parity ok: worst rel-L2 vs per-call {worst:.2e}, oracle {oracle_err}")
+    for _ in range(args.warmup):
+        ts.fused_step()
     torch.cuda.synchronize()
-    fused_out = [(L["Y"].clone(), L["dX"].clone(), L["dW"].clone()) for L in layers]
-    per_call("d")()
+    stream = torch.cuda.current_stream()
+    K = args.steps
+    # timed region: K steps back to back on one stream, an event pair around
+    # them, behind a queue backlog so the host's enqueue cost stays off the clock
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    for L, (y, dx, dw) in zip(layers, fused_out):
-        for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["dW"])):
-            rel = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
-            if rel > 1e-6:
-                raise RuntimeError(f"fused forward + backward differs from the per-call launches (rel-L2 {rel:.2e})")
-    plain_eager = time_eager(torch, per_call("t"), 20)
-    plain_graph = time_graph(torch, per_call("t"), 20)
-    dense_graph = time_graph(torch, per_call("d"), 20)
-    fused = time_eager(torch, bt.run, 50, backlog=True)
-    fused_graph = time_graph(torch, bt.run, 50)
-    repack_ms = time_graph(torch, repack, 20)
-    per_op = {lay: {op: time_graph(torch, per_call(lay, op), 20) for op in ("fwd", "dx", "dw")} for lay in ("t", "d")}
-    best_plain = min(plain_graph, dense_graph)
-    # library counterpart of the dW step: rocSPARSE SDDMM per layer (G rounded to FP16
-    # beforehand, the rounding the kernels apply on load)
-    lib_dw = lib_dx = None
-    if libs:
+    if hasattr(torch.cuda, "_sleep"):
+        torch.cuda._sleep(20_000_000)
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(K):
+        ts.fused_step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_per_step = t0.elapsed_time(t1) / K
+    # the persistent launch alone (the step's dominant kernel), K back to back
+    ts.batch.run(stage=1)
+    main_ms = time_eager(torch, lambda: ts.batch.run(stage=2), K, backlog=True)
+    prologue_ms = time_eager(torch, lambda: ts.batch.run(stage=1), K, backlog=True)
+    sgd_ms = time_eager(torch, ts.sgd, K, backlog=True)
+    graph_ms = time_graph(torch, ts.fused_step, K)
+    # the per-call paths, each with its own repack and the same SGD update
+    plain_graph = time_graph(torch, ts.per_call_step("t"), 20)
+    dense_graph = time_graph(torch, ts.per_call_step("d"), 20)
+    plain_eager = time_eager(torch, ts.per_call_step("t"), 10)
+    repack_t = time_graph(torch, lambda: ts.repack("t"), 20)
+    repack_d = time_graph(torch, lambda: ts.repack("d"), 20)
+    per_op = {lay: {op: time_graph(torch, (lambda lay=lay, op=op: ts.products(lay, op)), 20)
+                    for op in ("fwd", "dx", "dw")} for lay in ("t", "d")}
+    res = {
+        "ms_per_step": ms_per_step, "wall_s_timed": wall, "flops_per_step": ts.flops,
+        "dense_equivalent_flops_per_step": ts.dense_flops, "alg_bytes_per_step": ts.alg_bytes,
+        "main_launch_ms": main_ms, "prologue_ms": prologue_ms, "sgd_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
+        "plain_per_call_graph_ms": plain_graph, "plain_per_call_eager_ms": plain_eager,
+        "dense_per_call_graph_ms": dense_graph, "plain_repack_ms": repack_t, "dense_repack_ms": repack_d,
+        "speedup_vs_plain_per_call": plain_graph / ms_per_step,
+        "speedup_vs_dense_per_call": dense_graph / ms_per_step,
+        "speedup_vs_plain_per_call_eager": plain_eager / ms_per_step,
+        "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
+        "units": ts.batch.n_units, "workgroups": ts.batch.n_workgroups,
+        "parity": {"worst_rel_l2_vs_per_call": worst, "rel_l2_vs_fp64_oracle": oracle_err},
+        "layers": [{"name": L["case"].A.name, "nnz": L["case"].A.nnz, "N": L["N"], "fused": L["fused"],
+                    "per_call_tiled": L["t"]["choice"], "per_call_dense": L["d"]["choice"]} for L in ts.layers],
+    }
+    if not args.no_libs:
+        lib_dx = lib_dw = 0.0
         try:
-            lib_dx = 0.0
-            for L in layers:
-                W = L["sddmm_plan"]
-                ro = torch.from_numpy(L["row_offset"]).to(dev)
-                ci = torch.from_numpy(L["col_idx"]).to(dev)
-                out = torch.empty((W.n_cols, L["X"].shape[1]), dtype=torch.float32, device=dev)
-                lib_dx += T.rocsparse_spmm_t(W.n_rows, L["X"].shape[1], W.n_cols, ro, ci, L["w_vals"], L["G"].half(), out,
-                                             n_iter=10)
+            for L in ts.layers:
+                W = L["case"].A
+                ro = torch.from_numpy(W.row_offset).to(dev)
+                ci = torch.from_numpy(W.col_idx).to(dev)
+                vals = L["p16"].unpack(L["w"])
+                g16 = L["G"].half()
+                out = torch.empty((W.n_cols, L["N"]), dtype=torch.float32, device=dev)
+                lib_dx += T.rocsparse_spmm_t(W.n_rows, L["N"], W.n_cols, ro, ci, vals, g16, out, n_iter=5)
+                outw = torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)
+                lib_dw += T.rocsparse_sddmm(W.n_rows, W.n_cols, L["N"], ro, ci, g16, L["X"], outw, n_iter=5)
         except Exception as ex:  # report, do not hide
-            lib_dx = float("nan")
-            log("rocSPARSE transposed SpMM failed:", ex)
-        try:
-            lib_dw = 0.0
-            for L in layers:
-                W = L["sddmm_plan"]
-                ro = torch.from_numpy(L["row_offset"]).to(dev)
-                ci = torch.from_numpy(L["col_idx"]).to(dev)
-                out = torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)
-                lib_dw += T.rocsparse_sddmm(W.n_rows, W.n_cols, L["X"].shape[1], ro, ci, L["G"].half(), L["X"], out,
-                                            n_iter=10)
-        except Exception as ex:  # report, do not hide
-            lib_dw = float("nan")
-            log("rocSPARSE SDDMM failed:", ex)
-    return {"plain_per_call_eager_ms": plain_eager, "plain_per_call_graph_ms": plain_graph,
-            "dense_per_call_graph_ms": dense_graph, "fused_ms": fused, "fused_graph_ms": fused_graph,
-            "speedup_vs_graph": best_plain / fused, "speedup_vs_tiled_graph": plain_graph / fused,
-            "speedup_vs_eager": plain_eager / fused, "repack_ms": repack_ms,
-            "fused_tflops": flops / (fused * 1e-3) / 1e12, "fused_hbm_frac": nbytes / (fused * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
-            "rocsparse_dw_ms": lib_dw, "rocsparse_dx_ms": lib_dx,
-            "launches_plain": 3 * len(layers), "units": bt.n_units, "workgroups": bt.n_workgroups}
+            log("rocSPARSE backward legs failed:", ex)
+            lib_dx = lib_dw = float("nan")
+        res["rocsparse_dx_ms"] = lib_dx
+        res["rocsparse_dw_ms"] = lib_dw
+    return ts, res
 
 
-def bench_precision(torch, np, T, O, dev):
+# --------------------------------------------------------------------------
+# extra sections
+# --------------------------------------------------------------------------
+def bench_plain_gpu_suite(torch, np, T, O, dev, args):
+    """The ten FFN-shaped products (CSR 2048x512 and 512x2048 at sparsity
+    0.7..0.98, B K x 256): one batched launch, tuned per-call launches,
+    per-matrix warm/cold, Blocked-ELL, 2:4, rocSPARSE and hipBLASLt."""
+    from tca_spmm import workloads
+    from tca_spmm.workloads import alg_bytes_fwd
+    N = 256
+    suite = workloads.plain_gpu_suite(seed=0, N=N)
+
+    def f16(bits):
+        return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
+
+    entries = []
+    for case in suite:
+        A = case.A
+        vals = f16(A.values)
+        e = dict(case=case, B=f16(case.B), C=torch.empty((A.n_rows, N), dtype=torch.float32, device=dev),
+                 flops=2 * A.nnz * N, bytes=alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N))
+        for lay, dense in (("t", False), ("d", True)):
+            rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=dense)
+            plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
+            a = plan.pack(vals)
+            e[lay] = dict(plan=plan, a=plan.pack_dense(a) if dense else a, tile=tile, rpw=rpw, dense=dense)
+        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True, dense=True)
+        pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
+        e["b"] = dict(plan=pb, a=pb.pack_dense(pb.pack(vals)), tile=tile_b, rpw=rpw_b)
+        entries.append(e)
+    total_flops = sum(e["flops"] for e in entries)
+    total_bytes = sum(e["bytes"] for e in entries)
+
+    def launch(e, lay):
+        c = e[lay]
+        T.spmm(c["plan"], c["a"], e["B"], out=e["C"], tile_n=c["tile"], dense=c["dense"])
+
+    def per_call(lay):
+        def run():
+            for e in entries:
+                launch(e, lay)
+        return run
+
+    batch = T.Batch([T.spmm_item(e["b"]["plan"], e["b"]["a"], e["B"], e["C"], e["b"]["tile"], dense=True)
+                     for e in entries])
+    refs = [O.spmm_f32(e["case"].A.n_rows, N, e["case"].A.row_offset, e["case"].A.col_idx, e["case"].A.values,
+                       e["case"].B) for e in entries]
+    worst = 0.0
+    for run in (batch.run, per_call("t"), per_call("d")):
+        for e in entries:
+            e["C"].fill_(float("nan"))
+        run()
+        torch.cuda.synchronize()
+        for e, ref in zip(entries, refs):
+            r = rel_l2(np, e["C"].cpu().numpy(), ref)
+            worst = max(worst, r)
+            assert r <= 1e-5, f"{e['case'].A.name}: rel-L2 {r} vs oracle"
+    batch_ms = time_eager(torch, batch.run, 50, backlog=True)
+    out = {"batch_ms": batch_ms, "tflops": total_flops / (batch_ms * 1e-3) / 1e12,
+           "hbm_frac": total_bytes / (batch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes": total_bytes,
+           "worst_rel_l2_vs_oracle": worst,
+           "per_call_graph_ms": time_graph(torch, per_call("d"), 20),
+           "per_call_tiled_graph_ms": time_graph(torch, per_call("t"), 20),
+           "per_call_eager_ms": time_eager(torch, per_call("d"), 20),
+           "batch_units": batch.n_units}
+    per = []
+    for e in entries:
+        us = time_eager(torch, lambda: launch(e, "d"), 50, backlog=True) * 1e3
+        us_t = time_eager(torch, lambda: launch(e, "t"), 50, backlog=True) * 1e3
+        per.append({"name": e["case"].A.name, "us": us, "tiled_us": us_t, "tflops": e["flops"] / (us * 1e-6) / 1e12,
+                    "hbm_frac": e["bytes"] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                    "per_call_dense": [e["d"]["rpw"], e["d"]["tile"]], "per_call_tiled": [e["t"]["rpw"], e["t"]["tile"]]})
+    # cold: a 512 MiB write between launches evicts L2 and the Infinity Cache
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    for e, row in zip(entries, per):
+        cold = []
+        for _ in range(5):
+            flush.fill_(1)
+            s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            launch(e, "d")
+            t.record(stream)
+            torch.cuda.synchronize()
+            cold.append(s.elapsed_time(t) * 1e3)
+        row["cold_us"] = float(np.median(cold))
+    del flush
+    out["per_matrix"] = per
+    # Blocked-ELL (the reference's block-SpMM baseline, a gfx950 kernel here)
+    bell = {}
+    for b in (16, 32):
+        tot = 0.0
+        for e in entries:
+            A = e["case"].A
+            M_ = T.BlockedEll.from_csr(A.n_rows, A.n_cols, A.row_offset, A.col_idx, A.values, b)
+            Cb = torch.empty_like(e["C"])
+            tot += time_eager(torch, lambda: T.bell_spmm(M_, e["B"], out=Cb), 30, backlog=True)
+        bell[f"b{b}_ms"] = tot
+    out["blocked_ell"] = bell
+    # 2:4 on the sparse MFMA: each A pruned to 2:4 (a different, denser product)
+    tot, kept = 0.0, []
+    for e in entries:
+        A = e["case"].A
+        dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
+        rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
+        dense[rows, torch.from_numpy(A.col_idx).to(dev).long()] = f16(A.values)
+        sp = T.Sparse24.compress(dense)
+        C24 = torch.empty_like(e["C"])
+        tot += time_eager(torch, lambda: T.spmm_24(sp, e["B"], out=C24), 30, backlog=True)
+        kept.append(int((sp.values != 0).sum()) / max(A.nnz, 1))
+        del dense, sp
+    out["sparse24"] = {"ms": tot, "kept_nnz_frac_min": min(kept)}
+    if not args.no_libs:
+        lib_ms = {"rocsparse": 0.0, "hipblaslt": 0.0}
+        for e in entries:
+            A = e["case"].A
+            rp = torch.from_numpy(A.row_offset).to(dev)
+            ci = torch.from_numpy(A.col_idx).to(dev)
+            vv = f16(A.values)
+            Cl = torch.empty_like(e["C"])
+            try:
+                lib_ms["rocsparse"] += T.rocsparse_spmm(A.n_rows, N, A.n_cols, rp, ci, vv, e["B"], Cl, n_iter=20)
+            except Exception as ex:  # report, do not hide
+                lib_ms["rocsparse"] = float("nan")
+                log("rocSPARSE failed:", ex)
+            dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
+            rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
+            dense[rows, ci.long()] = vv
+            try:
+                lib_ms["hipblaslt"] += T.hipblaslt_gemm(A.n_rows, N, A.n_cols, dense, e["B"], Cl, n_iter=20)
+            except Exception as ex:
+                lib_ms["hipblaslt"] = float("nan")
+                log("hipBLASLt failed:", ex)
+        out["rocsparse_ms"] = lib_ms["rocsparse"]
+        out["hipblaslt_ms"] = lib_ms["hipblaslt"]
+    return out
+
+
+def bench_precision(torch, np, T, O, dev, args):
     """4096x4096 at 0.9, N=512: FP16, BF16 and FP8-E4M3 (row/column scales)
-    throughput, with rel-L2 / max-abs against the FP64 oracle on the original
-    values (format + kernel error) and on the quantised values (kernel error)."""
+    per call, with rel-L2 / max-abs against the FP64 oracle on the original
+    values (format + kernel error) and on the quantised values (kernel error);
+    hipBLASLt dense FP16 GEMM and rocSPARSE on the same matrix."""
     from tca_spmm import workloads
     from tca_spmm.workloads import alg_bytes_fwd
     case = workloads.precision(seed=0)
     A, N = case.A, case.N
     rp, ci = A.row_offset, A.col_idx
-    a16 = A.values.view(np.float16)
-    b16 = case.B.view(np.float16)
-    ref = O.spmm_f64(A.n_rows, N, rp, ci, a16.astype(np.float64), b16.astype(np.float64))
-    rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N)
-    plan = T.Plan(A.n_rows, A.n_cols, rp, ci, rpw)
+    ref = O.spmm_f64(A.n_rows, N, rp, ci, A.values.view(np.float16).astype(np.float64),
+                     case.B.view(np.float16).astype(np.float64))
     out = {}
 
     def err(C, r):
@@ -239,8 +488,11 @@ def bench_precision(torch, np, T, O, dev):
     va = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(dev)
     vb = torch.from_numpy(case.B.view(np.int16)).view(torch.float16).to(dev)
     C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
+    rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N)
     rpw_d, tile_d = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=True)
+    plan = T.Plan(A.n_rows, A.n_cols, rp, ci, rpw)
     plan_d = plan if rpw_d == rpw else T.Plan(A.n_rows, A.n_cols, rp, ci, rpw_d)
+    nbytes = alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N)
     for name, cast in (("f16", torch.float16), ("bf16", torch.bfloat16)):
         ta, tb = plan.pack(va.to(cast)), vb.to(cast)
         da = plan_d.pack_dense(plan_d.pack(va.to(cast)))
@@ -252,9 +504,11 @@ def bench_precision(torch, np, T, O, dev):
                         tb.float().cpu().numpy().astype(np.float64))
         e_all, m_all = err(Cn, ref)
         e_q, m_q = err(Cn, rq)
-        out[name] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
-                     "dense_tflops": 2 * A.nnz * N / (ms_d * 1e-3) / 1e12, "dense_rel_l2_vs_fp64": err(Cd, ref)[0],
-                     "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        best = min(ms, ms_d)
+        out[name] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "best_us": best * 1e3,
+                     "tflops": 2 * A.nnz * N / (best * 1e-3) / 1e12,
+                     "hbm_frac": nbytes / (best * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "dense_rel_l2_vs_fp64_on_quantised": err(Cd, rq)[0],
                      "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
                      "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q}
     rpd = torch.from_numpy(rp).to(dev)
@@ -263,7 +517,6 @@ def bench_precision(torch, np, T, O, dev):
     ta = plan.pack(qa)
     da = plan_d.pack_dense(plan_d.pack(qa))
     ms_d = time_graph(torch, lambda: T.spmm(plan_d, da, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile_d, dense=True), 50)
-    Cd = C.cpu().numpy()
     ms = time_graph(torch, lambda: T.spmm(plan, ta, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile), 50)
     Cn = C.cpu().numpy()
     rq = O.spmm_f64(A.n_rows, N, rp, ci, O.e4m3_to_f32(qa.cpu().numpy()).astype(np.float64),
@@ -271,28 +524,74 @@ def bench_precision(torch, np, T, O, dev):
     rq *= sa.cpu().numpy().astype(np.float64)[:, None] * sb.cpu().numpy().astype(np.float64)[None, :]
     e_all, m_all = err(Cn, ref)
     e_q, m_q = err(Cn, rq)
-    qms = time_graph(torch, lambda: T.quantize_cols_e4m3(vb), 20)
-    out["e4m3"] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "tflops": 2 * A.nnz * N / (ms * 1e-3) / 1e12,
-                   "dense_tflops": 2 * A.nnz * N / (ms_d * 1e-3) / 1e12, "dense_rel_l2_vs_fp64": err(Cd, ref)[0],
-                   "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N, 1, 1, True) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    best = min(ms, ms_d)
+    out["e4m3"] = {"us": ms * 1e3, "dense_us": ms_d * 1e3, "best_us": best * 1e3,
+                   "tflops": 2 * A.nnz * N / (best * 1e-3) / 1e12,
+                   "hbm_frac": alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N, 1, 1, True) / (best * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "rel_l2_vs_fp64": e_all, "max_abs_vs_fp64": m_all,
                    "rel_l2_vs_fp64_on_quantised": e_q, "max_abs_vs_fp64_on_quantised": m_q,
-                   "quantize_cols_us": qms * 1e3}
+                   "quantize_cols_us": time_graph(torch, lambda: T.quantize_cols_e4m3(vb), 20) * 1e3}
     out["config"] = {"rows_per_wave": rpw, "tile_n": tile, "dense_rows_per_wave": rpw_d, "dense_tile_n": tile_d,
-                     "nnz": A.nnz, "N": N}
+                     "nnz": A.nnz, "N": N, "alg_bytes_f16": nbytes}
+    if not args.no_libs:
+        dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
+        rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(rp)).to(dev))
+        dense[rows, torch.from_numpy(ci).to(dev).long()] = va
+        try:
+            ms_blt = T.hipblaslt_gemm(A.n_rows, N, A.n_cols, dense, vb, C, n_iter=50)
+            out["hipblaslt_dense_us"] = ms_blt * 1e3
+            out["speedup_vs_hipblaslt"] = ms_blt * 1e3 / out["f16"]["best_us"]
+        except Exception as ex:
+            log("hipBLASLt failed:", ex)
+        try:
+            ms_rs = T.rocsparse_spmm(A.n_rows, N, A.n_cols, rpd, torch.from_numpy(ci).to(dev), va, vb, C, n_iter=10)
+            out["rocsparse_us"] = ms_rs * 1e3
+        except Exception as ex:
+            log("rocSPARSE failed:", ex)
+        del dense
     return out
+
+
+def bench_plain_cpu(torch, np, T, O, dev, cores):
+    """512x512 at 0.9 (Binomial rows), N=256: the reference's CPU-runnable
+    case, on the GPU (tuned per-call launch) and on the CPU oracle (all
+    cores and one thread), same inputs."""
+    from tca_spmm import workloads
+    case = workloads.plain_cpu()
+    A, N = case.A, case.N
+    rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=True)
+    plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rpw)
+    va = torch.from_numpy(A.values.view(np.int16)).view(torch.float16).to(dev)
+    da = plan.pack_dense(plan.pack(va))
+    B = torch.from_numpy(case.B.view(np.int16)).view(torch.float16).to(dev)
+    C = torch.empty((A.n_rows, N), dtype=torch.float32, device=dev)
+    T.spmm(plan, da, B, out=C, tile_n=tile, dense=True)
+    ref = O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, case.B, cores)
+    r = rel_l2(np, C.cpu().numpy(), ref)
+    assert r <= 1e-5, f"plain_cpu rel-L2 {r}"
+    gpu_us = time_eager(torch, lambda: T.spmm(plan, da, B, out=C, tile_n=tile, dense=True), 100, backlog=True) * 1e3
+    res = {"gpu_us": gpu_us, "rel_l2_gpu_vs_cpu": r, "flops": 2 * A.nnz * N}
+    for th, key in ((cores, "cpu_all_cores"), (1, "cpu_one_thread")):
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 1.0:
+            O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, case.B, th)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        res[key] = {"us": dt * 1e6, "threads": th, "gflops": 2 * A.nnz * N / dt / 1e9}
+    res["gpu_speedup_vs_cpu_all_cores"] = res["cpu_all_cores"]["us"] / gpu_us
+    return res
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="fwdbwd,precision",
-                    help="comma list of extra sections: fwdbwd, precision")
+    ap.add_argument("--sections", default="suite,precision,plain_cpu",
+                    help="comma list of extra sections: suite, precision, plain_cpu")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-libs", action="store_true", help="skip rocSPARSE / hipBLASLt")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 10x (HBM counter calibration)")
     args = ap.parse_args()
@@ -317,188 +616,22 @@ def main():
     import torch
 
     import tca_spmm as T
-    from tca_spmm import workloads
-    from tca_spmm.workloads import alg_bytes_fwd
+    from oracle import oracle as O
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    N = 256
-    suite = workloads.plain_gpu_suite(seed=0, N=N)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
 
-    def f16(bits):
-        return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
-
-    entries = []
-    for case in suite:
-        A = case.A
-        vals = f16(A.values)
-        e = dict(case=case, B=f16(case.B), C=torch.empty((A.n_rows, N), dtype=torch.float32, device=dev),
-                 flops=2 * A.nnz * N, bytes=alg_bytes_fwd(A.n_rows, A.n_cols, A.nnz, N))
-        # per-call launches: each product's tuned (rows_per_wave, tile_n), with A
-        # as tiled values ("t") and as panel-dense tiles ("d")
-        for lay, dense in (("t", False), ("d", True)):
-            rpw, tile = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, dense=dense)
-            plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw)
-            a = plan.pack(vals)
-            e[lay] = dict(plan=plan, a=plan.pack_dense(a) if dense else a, tile=tile, rpw=rpw, dense=dense)
-        # the step's batch item: panel-dense tiles of a 32-row plan, 64-column tiles
-        rpw_b, tile_b = T.autotune.choice_for(A.n_rows, A.n_cols, A.nnz, N, batch=True, dense=True)
-        pb = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rows_per_wave=rpw_b)
-        e["b"] = dict(plan=pb, a=pb.pack_dense(pb.pack(vals)), tile=tile_b, rpw=rpw_b)
-        entries.append(e)
-    total_flops = sum(e["flops"] for e in entries)
-    total_bytes = sum(e["bytes"] for e in entries)
-
-    def launch(e, lay):
-        c = e[lay]
-        T.spmm(c["plan"], c["a"], e["B"], out=e["C"], tile_n=c["tile"], dense=c["dense"])
-
-    def per_call(lay):
-        def run():
-            for e in entries:
-                launch(e, lay)
-        return run
-
-    # The step: the ten products as ONE persistent batched launch (their work
-    # units drain through one grid), the fastest way found to run them.
-    batch = T.Batch([T.spmm_item(e["b"]["plan"], e["b"]["a"], e["B"], e["C"], e["b"]["tile"], dense=True)
-                     for e in entries])
-
-    def step():
-        batch.run()
-
-    # parity gate before timing: every product of the step, and of the per-call
-    # launches, against the CPU oracle
-    from oracle import oracle as O
-    refs = []
-    for e in entries:
-        A = e["case"].A
-        refs.append(O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B))
-    worst = 0.0
-    for run in (step, per_call("t"), per_call("d")):
-        for e in entries:
-            e["C"].fill_(float("nan"))
-        run()
-        torch.cuda.synchronize()
-        for e, ref in zip(entries, refs):
-            rel = float(np.linalg.norm(e["C"].cpu().numpy() - ref) / np.linalg.norm(ref))
-            worst = max(worst, rel)
-            assert rel <= 1e-5, f"{e['case'].A.name}: rel-L2 {rel} vs oracle"
-    log(f"parity ok: worst rel-L2 vs CPU oracle {worst:.2e}")
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-
-    # timed region: K steps back to back on one stream, an event pair around
-    # them, behind a queue backlog so the host's enqueue cost stays off the GPU
-    # clock.  A step is one kernel launch, so no graph is needed to hide launch
-    # cost; the same step replayed from a HIP graph is reported beside it.
-    K = args.steps
-    t_all0, t_all1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    if hasattr(torch.cuda, "_sleep"):
-        torch.cuda._sleep(20_000_000)
-    wall0 = time.perf_counter()
-    t_all0.record(stream)
-    for _ in range(K):
-        step()
-    t_all1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms_per_step = t_all0.elapsed_time(t_all1) / K
-    value = total_flops / (ms_per_step * 1e-3) / 1e12
-    # roofline of the step's kernel (batch_kernel, one launch per step): the
-    # ten products' algorithmic bytes / the launch's duration
-    launch_us = ms_per_step * 1e3
-    achieved = total_bytes / (ms_per_step * 1e-3) / 1e9
-    graph_ms = time_graph(torch, step, K)
-
-    # the ten tuned per-call launches, captured in one graph (one stream) and
-    # eager, with dense tiles and with tiled values
-    per_call_graph_ms = time_graph(torch, per_call("d"), K)
-    eager_ms = time_eager(torch, per_call("d"), K)
-    tiled_graph_ms = time_graph(torch, per_call("t"), K)
-
-    extra = {"graph_replay_ms_per_step": graph_ms, "wall_s_timed": wall,
-             "batch_units": batch.n_units, "batch_workgroups": batch.n_workgroups,
-             "per_call_graph_ms_per_step": per_call_graph_ms, "per_call_eager_ms_per_step": eager_ms,
-             "per_call_tiled_graph_ms_per_step": tiled_graph_ms,
-             "speedup_vs_per_call_graph": per_call_graph_ms / ms_per_step,
-             "configs": [{"name": e["case"].A.name, "nnz": e["case"].A.nnz,
-                          "per_call_dense": [e["d"]["rpw"], e["d"]["tile"]], "per_call_tiled": [e["t"]["rpw"], e["t"]["tile"]],
-                          "batch": [e["b"]["rpw"], e["b"]["tile"]]} for e in entries]}
-
-    # per-matrix times of the tuned per-call launch (dense tiles; tiled values
-    # beside it): 50 back-to-back launches after a queue backlog
-    per = []
-    for e in entries:
-        us = time_eager(torch, lambda: launch(e, "d"), 50, backlog=True) * 1e3
-        us_t = time_eager(torch, lambda: launch(e, "t"), 50, backlog=True) * 1e3
-        per.append({"name": e["case"].A.name, "us": us, "tiled_us": us_t, "tflops": e["flops"] / (us * 1e-6) / 1e12,
-                    "hbm_frac": e["bytes"] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS})
-    # cold: a 512 MiB write between launches evicts L2 and the Infinity Cache, so
-    # every operand comes from HBM (event pair around the launch alone)
-    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
-    for e, row in zip(entries, per):
-        cold = []
-        for _ in range(10):
-            flush.fill_(1)
-            s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s.record(stream)
-            launch(e, "d")
-            t.record(stream)
-            torch.cuda.synchronize()
-            cold.append(s.elapsed_time(t) * 1e3)
-        row["cold_us"] = float(np.median(cold))
-    del flush
-    extra["per_matrix"] = per
-
-    # Blocked-ELL (the reference's block-SpMM baseline, a gfx950 kernel here) on
-    # the same ten products, blocks of 16 and 32: the format pads every block
-    # row to the longest one and every kept block to b x b, so it moves and
-    # multiplies `fill` times the nonzeros
-    bell = {"rows": [], "ms_per_step": {}}
-    for b in (16, 32):
-        tot = 0.0
-        for i, e in enumerate(entries):
-            A = e["case"].A
-            M_ = T.BlockedEll.from_csr(A.n_rows, A.n_cols, A.row_offset, A.col_idx, A.values, b)
-            Cb = torch.empty_like(e["C"])
-            us = time_eager(torch, lambda: T.bell_spmm(M_, e["B"], out=Cb), 50, backlog=True) * 1e3
-            launch(e, "d")
-            torch.cuda.synchronize()
-            rel = float(torch.linalg.norm(Cb - e["C"]) / torch.linalg.norm(e["C"]))
-            if b == 16:
-                bell["rows"].append({"name": A.name})
-            bell["rows"][i][f"b{b}_us"] = us
-            bell["rows"][i][f"b{b}_fill"] = M_.values.numel() / max(A.nnz, 1)
-            bell["rows"][i][f"b{b}_rel_vs_spmm"] = rel
-            tot += us
-        bell["ms_per_step"][f"b{b}"] = tot * 1e-3
-    extra["blocked_ell"] = bell
-
-    # 2:4 (the reference's cuSPARSELt comparison, here gfx950's sparse MFMA): each
-    # product's A densified, pruned to 2:4 and compressed outside the timed
-    # region -- a different, denser product than the SpMM's
-    s24 = {"rows": []}
-    tot = 0.0
-    for e in entries:
-        A = e["case"].A
-        dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
-        rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
-        dense[rows, torch.from_numpy(A.col_idx).to(dev).long()] = f16(A.values)
-        sp = T.Sparse24.compress(dense)
-        C24 = torch.empty_like(e["C"])
-        us = time_eager(torch, lambda: T.spmm_24(sp, e["B"], out=C24), 50, backlog=True) * 1e3
-        kept = int((sp.values != 0).sum())
-        s24["rows"].append({"name": A.name, "us": us, "kept_nnz_frac": kept / max(A.nnz, 1)})
-        tot += us
-        del dense, sp
-    s24["ms_per_step"] = tot * 1e-3
-    extra["sparse24"] = s24
+    ts, step = bench_train_step(torch, np, T, O, dev, args)
+    log(f"train step {step['ms_per_step'] * 1e3:.1f} us, plain per-call {step['plain_per_call_graph_ms'] * 1e3:.1f} us")
+    ms_per_step = step["ms_per_step"]
+    value = ts.flops / (ms_per_step * 1e-3) / 1e12
+    main_ms = step["main_launch_ms"]
+    achieved = ts.alg_bytes / (main_ms * 1e-3) / 1e9
 
     # achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
     src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
@@ -508,70 +641,42 @@ def main():
     copy_gbs = 2 * src.numel() / (copy_ms * 1e-3) / 1e9      # bytes read + written
     del src, dst
 
-    # library baselines on the same inputs
-    if not args.no_libs:
-        lib_ms = {"rocsparse": 0.0, "hipblaslt": 0.0}
-        for e in entries:
-            A = e["case"].A
-            rp = torch.from_numpy(A.row_offset).to(dev)
-            ci = torch.from_numpy(A.col_idx).to(dev)
-            vv = f16(A.values)
-            Cl = torch.empty_like(e["C"])
-            try:
-                lib_ms["rocsparse"] += T.rocsparse_spmm(A.n_rows, N, A.n_cols, rp, ci, vv, e["B"], Cl, n_iter=20)
-            except Exception as ex:  # report, do not hide
-                lib_ms["rocsparse"] = float("nan")
-                log("rocSPARSE failed:", ex)
-            dense = torch.zeros((A.n_rows, A.n_cols), dtype=torch.float16, device=dev)
-            rows = torch.repeat_interleave(torch.arange(A.n_rows, device=dev), torch.from_numpy(np.diff(A.row_offset)).to(dev))
-            dense[rows, ci.long()] = vv
-            try:
-                lib_ms["hipblaslt"] += T.hipblaslt_gemm(A.n_rows, N, A.n_cols, dense, e["B"], Cl, n_iter=20)
-            except Exception as ex:
-                lib_ms["hipblaslt"] = float("nan")
-                log("hipBLASLt failed:", ex)
-        extra["rocsparse_ms_per_step"] = lib_ms["rocsparse"]
-        extra["hipblaslt_ms_per_step"] = lib_ms["hipblaslt"]
-        extra["speedup_vs_rocsparse"] = lib_ms["rocsparse"] / ms_per_step if lib_ms["rocsparse"] else None
-        extra["speedup_vs_hipblaslt"] = lib_ms["hipblaslt"] / ms_per_step if lib_ms["hipblaslt"] else None
-
-    if "fwdbwd" in args.sections:
-        extra["fwd_bwd_12_layers"] = bench_fwd_bwd(torch, np, T, O, dev, libs=not args.no_libs)
+    extra = {"train_step": step}
+    if "suite" in args.sections:
+        extra["plain_gpu_suite"] = bench_plain_gpu_suite(torch, np, T, O, dev, args)
     if "precision" in args.sections:
-        extra["precision_4096"] = bench_precision(torch, np, T, O, dev)
+        extra["precision_4096"] = bench_precision(torch, np, T, O, dev, args)
+    if "plain_cpu" in args.sections:
+        extra["plain_cpu_512"] = bench_plain_cpu(torch, np, T, O, dev, cores)
 
     cpu = None
     if not args.no_cpu:
-        try:
-            cores = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            cores = min(16, os.cpu_count() or 1)
+        ts.cpu_step(O, cores)                         # warm-up (and the transposes)
         reps, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < args.cpu_seconds:
-            for e in entries:
-                A = e["case"].A
-                O.spmm_f32(A.n_rows, N, A.row_offset, A.col_idx, A.values, e["case"].B, cores)
+            ts.cpu_step(O, cores)
             reps += 1
         dt = time.perf_counter() - t0
-        # one thread on the first matrix, for the scaling of the CPU figure
-        A0 = entries[0]["case"].A
+        cpu = {"value": ts.flops * reps / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+               "sample": f"the full twelve-layer step's products (forward, dX on the transposed CSR, dW; FP32 "
+                         f"accumulation of FP16 products, OpenMP over rows, {cores} threads = the process's "
+                         f"affinity set) repeated {reps}x in {dt:.1f}s ({dt / reps * 1e3:.1f} ms per step)"}
+        L0 = ts.layers[0]
+        W0 = L0["case"].A
         reps1, t1 = 0, time.perf_counter()
         while time.perf_counter() - t1 < 2.0:
-            O.spmm_f32(A0.n_rows, N, A0.row_offset, A0.col_idx, A0.values, entries[0]["case"].B, 1)
+            O.spmm_f32(W0.n_rows, L0["N"], W0.row_offset, W0.col_idx, W0.values, L0["case"].B, 1)
             reps1 += 1
         dt1 = time.perf_counter() - t1
-        extra["cpu_one_thread"] = {"matrix": A0.name, "ms": dt1 / reps1 * 1e3,
-                                   "tflops": entries[0]["flops"] * reps1 / dt1 / 1e12}
-        cpu = {"value": total_flops * reps / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-               "sample": f"the same ten-matrix suite (N=256) repeated {reps}x in {dt:.1f}s "
-                         f"({dt / reps * 1e3:.2f} ms per suite), FP32 accumulate, OpenMP over rows"}
+        extra["cpu_one_thread"] = {"product": W0.name + " forward", "ms": dt1 / reps1 * 1e3,
+                                   "tflops": 2 * W0.nnz * L0["N"] * reps1 / dt1 / 1e12}
 
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "TFLOP/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -579,17 +684,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f16",
         "data": "synthetic",
-        "config": {"workload": "plain_gpu: CSR 2048x512 and 512x2048 at sparsity 0.7/0.8/0.9/0.95/0.98 "
-                               "(lognormal(0.5) rows), B Kx256, FP16 in, FP32 accumulate/out; 10 SpMMs per step "
-                               "as one persistent batched launch, A as panel-dense 16x32 tiles",
-                   "N": N, "matrices": len(entries), "nnz_total": int(sum(e["case"].A.nnz for e in entries)),
-                   "parallelism": "single GPU"},
+        "config": {"workload": "twelve-layer sparse-weight training step: 4x 512x2048, 4x 2048x512, 2x 1024x1024, "
+                               "2x 4096x1024 CSR layers, sparsity U[0.5,0.98], lognormal(0.5) rows, N in "
+                               "{128,384,1024}; SGD update + forward + dX + dW per step (fused: prologue launch "
+                               "packing dense tiles from the updated weights + one persistent launch)",
+                   "layers": len(ts.layers), "nnz_total": int(sum(L["case"].A.nnz for L in ts.layers)),
+                   "flops_per_step": ts.flops, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch_plain_gpu_suite_traffic"),
-                     "kernel": "batch_kernel (the step's one launch: ten products' algorithmic bytes / the "
-                               "launch's duration, event pair around K back-to-back launches)",
-                     "alg_bytes_per_step": total_bytes, "launch_us_mean": launch_us,
-                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch_train_step_traffic"),
+                     "kernel": "batch_kernel (the step's persistent launch: the twelve layers' forward + dX + dW "
+                               "algorithmic bytes / the launch's mean duration, K back-to-back launches)",
+                     "alg_bytes_per_launch": ts.alg_bytes, "launch_us_mean": main_ms * 1e3,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "mfma_dense_equivalent_tflops": ts.dense_flops / (main_ms * 1e-3) / 1e12,
+                     "mfma_peak_tflops": MFMA_F16_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "extra": extra,
     }

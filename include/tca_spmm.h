@@ -273,8 +273,16 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 /* 64-row (32) plans, or an F32 B on 16-row (16/32/64) or 32-row (32) plans. */
 /* SDDMM items: F16 X, F32 or F16 G.  Other combinations: UNSUPPORTED.       */
 /* An F32 operand (SpMM B, SDDMM G) gets a batch-owned F16 copy (2 bytes per  */
-/* element) that every run refreshes first -- one extra kernel, same bits as  */
-/* rounding on load -- and the items read the copy: half the gather bytes.   */
+/* element) that every run refreshes first -- same bits as rounding on load -- */
+/* and the items read the copy: half the gather bytes.                       */
+/* a_dense = 2 (SPMM, F16): `a` holds the weight's VALUES, in the CSR order  */
+/* of the forward matrix (values_order NULL) or in the tiled order of          */
+/* values_order (a forward plan of the same pattern, e.g. the SDDMM item's    */
+/* plan, so that dW and the weight share one order); the batch owns the        */
+/* item's panel-dense tiles and rebuilds them from those values at the start */
+/* of every run.  A training step that updates the weight in place then needs */
+/* no separate repack: every run = one prologue launch (FP16 copies + dense   */
+/* tiles of all such items) + the persistent launch.                          */
 /* ------------------------------------------------------------------------ */
 typedef enum tca_op { TCA_OP_SPMM = 0, TCA_OP_SDDMM = 1 } tca_op;
 typedef struct tca_batch_item {
@@ -290,11 +298,18 @@ typedef struct tca_batch_item {
     int N;
     float *out;            /* SPMM: C (M x N, ldc);     SDDMM: out_tiled     */
     int ldc;
-    int a_dense;           /* SPMM: 1 = `a` holds panel-dense tiles (tca_pack_dense) */
+    int a_dense;           /* SPMM: 1 = `a` holds panel-dense tiles (tca_pack_dense);
+                              2 = `a` holds values the batch packs every run (above) */
+    const tca_plan *values_order;   /* a_dense = 2: order of `a` (NULL = CSR order) */
 } tca_batch_item;
 typedef struct tca_batch tca_batch;
 tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch);
 tca_status tca_batch_run(tca_batch *batch, void *stream);
+/* One stage of tca_batch_run: 1 = the prologue (FP16 copies of F32 operands and
+ * dense tiles of a_dense = 2 items; nothing is launched when the batch has
+ * neither), 2 = the persistent launch over the work units, 3 = both.  Lets a
+ * caller time the persistent kernel alone. */
+tca_status tca_batch_run_stage(tca_batch *batch, int stage, void *stream);
 tca_status tca_batch_destroy(tca_batch *batch);
 /* Work units of a batch and the number of workgroups its launch uses. */
 tca_status tca_batch_get_info(const tca_batch *batch, int *n_units, int *n_workgroups);

@@ -352,6 +352,44 @@ void orc_sddmm_f64(int M, int N, const int *rowptr, const int *colidx,
     }
 }
 
+/* FP32 SDDMM on fp16 bits: out[e] = sum_n f32(G[r,n]) * f32(X[col(e),n]),
+ * FP32 accumulation in ascending n (the CPU baseline of the dW product; the
+ * reference has no backward, so this follows the same naive_gemm semantics,
+ * src/miscalleneous.cu:114-134, restricted to the pattern). */
+void orc_sddmm_f32(int M, int N, const int *rowptr, const int *colidx, const uint16_t *G, int ldg,
+                   const uint16_t *X, int ldx, float *out, int nthreads)
+{
+    int K = 0;
+    for (int e = 0; e < rowptr[M]; e++) if (colidx[e] + 1 > K) K = colidx[e] + 1;
+    float *Xf = (float *)malloc(sizeof(float) * ((size_t)K * N + 1));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (int k = 0; k < K; k++)
+        for (int n = 0; n < N; n++) Xf[(size_t)k * N + n] = orc_f16_to_f32(X[(size_t)k * ldx + n]);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        float *g = (float *)malloc(sizeof(float) * ((size_t)N + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int r = 0; r < M; r++) {
+            for (int n = 0; n < N; n++) g[n] = orc_f16_to_f32(G[(size_t)r * ldg + n]);
+            for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+                const float *x = Xf + (size_t)colidx[e] * N;
+                float s = 0.0f;
+                for (int n = 0; n < N; n++) s += g[n] * x[n];
+                out[e] = s;
+            }
+        }
+        free(g);
+    }
+    free(Xf);
+}
+
 /* ------------------------------------------------------------------------ */
 /* workload_manager -- src/our_spmm.cu:347-471 (row -> warp packing)          */
 /* ------------------------------------------------------------------------ */

@@ -19,6 +19,8 @@
  *                            pattern (C == B), all-ones values (row sums), dense
  *                            numpy matmul on small shapes, f32 vs f64 bound.
  *   orc_sddmm_f64         -- dense numpy G @ X.T sampled on the pattern.
+ *   orc_sddmm_f32         -- exact on small-integer operands (== numpy), a
+ *                            dropped or transposed term fails; FP32 vs FP64 bound.
  *   orc_workload_manager  -- known answer on the bundled file at N=256:
  *                            24 blocks, 2 idle warp slots, 0 duplicates.
  *   orc_tiling_*          -- CSR reconstruction identity + brute-force union.
@@ -80,6 +82,9 @@ void orc_spmm_f64(int M, int N, const int *rowptr, const int *colidx, const doub
 /* Sampled dense-dense product on the CSR pattern: out[e] = sum_n G[i,n] X[j,n]. */
 void orc_sddmm_f64(int M, int N, const int *rowptr, const int *colidx,
                    const double *G, int ldg, const double *X, int ldx, double *out, int nthreads);
+/* The same on fp16 bits with FP32 accumulation in ascending n (CPU baseline of dW). */
+void orc_sddmm_f32(int M, int N, const int *rowptr, const int *colidx, const uint16_t *G, int ldg,
+                   const uint16_t *X, int ldx, float *out, int nthreads);
 
 /* ---- reference row->warp packing: src/our_spmm.cu:347-471 ---------------- */
 /* warps_per_row[M]; row_idx/warpid capacity `cap` entries.  Returns the number

@@ -65,6 +65,9 @@ def lib():
         L.orc_sddmm_f64.restype = None
         L.orc_sddmm_f64.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_int,
                                     _f64p, ctypes.c_int, _f64p, ctypes.c_int]
+        L.orc_sddmm_f32.restype = None
+        L.orc_sddmm_f32.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _u16p, ctypes.c_int,
+                                    _u16p, ctypes.c_int, _f32p, ctypes.c_int]
         L.orc_workload_manager.restype = ctypes.c_int
         L.orc_workload_manager.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _i32p,
                                            _i32p, _i32p, _i32p, ctypes.c_int]
@@ -236,6 +239,20 @@ def sddmm_f64(M, N, rowptr, col, G, X, nthreads=0):
     lib().orc_sddmm_f64(M, N, _p(rowptr, _i32p), _p(col, _i32p), _p(Gd, _f64p), Gd.shape[1],
                         _p(Xd, _f64p), Xd.shape[1], _p(out, _f64p), nthreads)
     return out[:col.size]
+
+
+def sddmm_f32(M, N, rowptr, col, G_bits, X_bits, nthreads=0):
+    """FP32-accumulate CPU SDDMM on fp16 bits (ascending n): out[e] = G[r] . X[col(e)]."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    Gb = np.ascontiguousarray(G_bits, np.uint16)
+    Xb = np.ascontiguousarray(X_bits, np.uint16)
+    out = np.empty(max(col.size, 1), np.float32)
+    if col.size == 0:
+        col = np.zeros(1, np.int32)
+    lib().orc_sddmm_f32(M, N, _p(rowptr, _i32p), _p(col, _i32p), _p(Gb, _u16p), Gb.shape[1],
+                        _p(Xb, _u16p), Xb.shape[1], _p(out, _f32p), nthreads)
+    return out[:int(rowptr[-1])]
 
 
 def workload_manager(rowptr, N, warps_per_tb=8):

@@ -25,6 +25,12 @@ struct tca_batch {
     tca::ConvDesc *d_conv = nullptr;
     int n_conv = 0;
     long conv8 = 0;
+    // panel-dense tiles of a_dense = 2 items, rebuilt from the weight values on
+    // every run (the same prologue launch as the FP16 copies)
+    void *d_pack_mem = nullptr;  // pack descriptors | dense tiles | value maps
+    tca::PackDesc *d_pack = nullptr;
+    int n_pack = 0;
+    long pack_blocks = 0;
 };
 
 namespace {
@@ -114,8 +120,89 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         }
         if (hipMemcpy(shadow.p, conv.data(), sizeof(tca::ConvDesc) * conv.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
-        items = its.data();
     }
+    // Items whose weight values the batch packs into panel-dense tiles on every
+    // run: map each entry of the item's plan to its value's index in `a`.
+    struct PackJob {
+        int item;
+        std::vector<int32_t> map;   // empty: identity
+        size_t tiles_off = 0, map_off = 0;
+        long n_tiles = 0;
+    };
+    std::vector<PackJob> jobs;
+    size_t pack_bytes = 0;
+    for (int i = 0; i < n_items; i++) {
+        const tca_batch_item &it = its[i];
+        if (it.a_dense != 2) continue;
+        const tca_plan *p = it.plan;
+        const std::string who = "tca_batch_create: item " + std::to_string(i);
+        if (it.op != TCA_OP_SPMM || !p || !p->d_base)
+            return fail(TCA_ERR_INVALID_ARG, who + ": a_dense = 2 needs an SPMM item with a device plan");
+        if (it.a_dtype != TCA_F16) return fail(TCA_ERR_UNSUPPORTED, who + ": a_dense = 2 needs F16 values");
+        const tca_plan *q = it.values_order;
+        if (q && (q->transposed || q->pattern_key != p->pattern_key || q->host.nnz != p->host.nnz))
+            return fail(TCA_ERR_INVALID_ARG, who + ": values_order must be a forward plan of the item's pattern");
+        if (p->host.nnz > 0 && !it.a) return fail(TCA_ERR_INVALID_ARG, who + ": NULL values");
+        PackJob j;
+        j.item = i;
+        j.n_tiles = (long)p->host.n_chunks * (p->host.rpw / 16);
+        if (q != p) {
+            const int nnz = p->host.nnz;
+            const std::vector<int32_t> &csr = p->transposed ? p->entry_csr : p->host.entry_src;
+            j.map.resize(nnz);
+            if (q) {
+                std::vector<int32_t> tiled_of_csr(nnz);
+                for (int e = 0; e < nnz; e++) tiled_of_csr[q->host.entry_src[e]] = e;
+                for (int e = 0; e < nnz; e++) j.map[e] = tiled_of_csr[csr[e]];
+            } else {
+                for (int e = 0; e < nnz; e++) j.map[e] = csr[e];
+            }
+        }
+        jobs.push_back(std::move(j));
+    }
+    if ((int)jobs.size() > tca::kMaxPack) return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: too many packed items");
+    std::vector<tca::PackDesc> packs;
+    long pack_blocks = 0;
+    struct DevGuard2 {
+        void *p = nullptr;
+        ~DevGuard2()
+        {
+            if (p) (void)hipFree(p);
+        }
+    } packmem;
+    if (!jobs.empty()) {
+        pack_bytes = (sizeof(tca::PackDesc) * jobs.size() + 255) / 256 * 256;
+        for (PackJob &j : jobs) {
+            j.tiles_off = pack_bytes;
+            pack_bytes += ((size_t)j.n_tiles * 1024 + 255) / 256 * 256;
+            j.map_off = pack_bytes;
+            pack_bytes += (j.map.size() * 4 + 255) / 256 * 256;
+        }
+        if (hipMalloc(&packmem.p, pack_bytes) != hipSuccess) return fail(TCA_ERR_ALLOC, "tca_batch_create: dense tiles");
+        char *base = static_cast<char *>(packmem.p);
+        for (PackJob &j : jobs) {
+            tca_batch_item &it = its[j.item];
+            const tca_plan *p = it.plan;
+            if (!j.map.empty() &&
+                hipMemcpy(base + j.map_off, j.map.data(), j.map.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
+            tca::PackDesc d{};
+            d.seg_ptr = p->d_seg_ptr;
+            d.entry_slot = p->d_entry_slot;
+            d.map = j.map.empty() ? nullptr : reinterpret_cast<const int32_t *>(base + j.map_off);
+            d.src = static_cast<const uint16_t *>(it.a);
+            d.dst = reinterpret_cast<uint16_t *>(base + j.tiles_off);
+            d.n_tiles = j.n_tiles;
+            d.first_block = pack_blocks;
+            pack_blocks += (j.n_tiles + tca::kPackTilesPerBlock - 1) / tca::kPackTilesPerBlock;
+            packs.push_back(d);
+            it.a = d.dst;    // the items read the batch's tiles
+            it.a_dense = 1;
+        }
+        if (hipMemcpy(packmem.p, packs.data(), sizeof(tca::PackDesc) * packs.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
+    }
+    items = its.data();
     std::vector<tca::BatchItemDev> dev(n_items);
     std::vector<tca::BatchUnitDev> units;
     std::vector<double> cost;
@@ -320,12 +407,21 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     b->d_conv = static_cast<tca::ConvDesc *>(shadow.p);
     b->d_shadow = shadow.p;
     shadow.p = nullptr;   // owned by the batch from here
+    b->d_pack_mem = packmem.p;
+    b->d_pack = static_cast<tca::PackDesc *>(packmem.p);
+    b->n_pack = (int)packs.size();
+    b->pack_blocks = pack_blocks;
+    packmem.p = nullptr;
     const size_t bi = sizeof(tca::BatchItemDev) * dev.size();
     const size_t bu = sizeof(tca::BatchUnitDev) * std::max<size_t>(sorted.size(), 1);
     const size_t bz = sizeof(tca::ZeroDesc) * std::max<size_t>(zero.size(), 1);
     const size_t bc = (bi + bu + bz + 255) / 256 * 256;   // counters start on a 256-byte line
     const size_t total = bc + tca::kBatchCounterBytes + 256;   // + partition offsets
-    if (hipMalloc(&b->d_mem, total) != hipSuccess) { delete b; return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation"); }
+    if (hipMalloc(&b->d_mem, total) != hipSuccess) {
+        b->d_mem = nullptr;
+        tca_batch_destroy(b);
+        return fail(TCA_ERR_ALLOC, "tca_batch_create: device allocation");
+    }
     char *m = static_cast<char *>(b->d_mem);
     b->d_items = reinterpret_cast<tca::BatchItemDev *>(m);
     b->d_units = reinterpret_cast<tca::BatchUnitDev *>(m + bi);
@@ -346,23 +442,29 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     return TCA_OK;
 }
 
-extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream)
+extern "C" tca_status tca_batch_run_stage(tca_batch *batch, int stage, void *stream)
 {
     if (!batch) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run: NULL batch");
-    if (batch->n_conv) {
-        const tca_status s = tca::launch_convert_f16(batch->d_conv, batch->n_conv, batch->conv8, static_cast<hipStream_t>(stream));
+    if (stage < 1 || stage > 3) return fail(TCA_ERR_INVALID_ARG, "tca_batch_run_stage: stage must be 1, 2 or 3");
+    if ((stage & 1) && (batch->n_conv || batch->n_pack)) {
+        const tca_status s = tca::launch_prologue(batch->d_conv, batch->n_conv, batch->conv8, batch->d_pack,
+                                                  batch->n_pack, batch->pack_blocks, static_cast<hipStream_t>(stream));
         if (s != TCA_OK) return s;
     }
+    if (!(stage & 2)) return TCA_OK;
     return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->d_part_off,
                              batch->nparts, batch->grid, batch->d_zero, batch->n_zero, batch->dense_only,
                              static_cast<hipStream_t>(stream));
 }
+
+extern "C" tca_status tca_batch_run(tca_batch *batch, void *stream) { return tca_batch_run_stage(batch, 3, stream); }
 
 extern "C" tca_status tca_batch_destroy(tca_batch *batch)
 {
     if (!batch) return TCA_OK;
     if (batch->d_mem) (void)hipFree(batch->d_mem);
     if (batch->d_shadow) (void)hipFree(batch->d_shadow);
+    if (batch->d_pack_mem) (void)hipFree(batch->d_pack_mem);
     delete batch;
     return TCA_OK;
 }

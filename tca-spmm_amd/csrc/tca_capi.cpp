@@ -78,6 +78,21 @@ int dtype_bytes(tca_dtype d)
 
 bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
+// FNV-1a over the shape and the CSR arrays: identifies a pattern across plans
+uint64_t pattern_digest(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    const int dims[3] = {n_rows, n_cols, nnz};
+    mix(dims, sizeof(dims));
+    mix(row_offset, sizeof(int32_t) * ((size_t)n_rows + 1));
+    if (nnz > 0) mix(col_idx, sizeof(int32_t) * (size_t)nnz);
+    return h;
+}
+
 }  // namespace
 
 namespace {
@@ -94,6 +109,7 @@ tca_status plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offse
     if (s != TCA_OK) { delete p; return s; }
     p->csr_row_offset.assign(row_offset, row_offset + n_rows + 1);
     p->csr_col_idx.assign(col_idx, col_idx + nnz);
+    p->pattern_key = pattern_digest(n_rows, n_cols, nnz, row_offset, col_idx);
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (device) {
         s = upload(p);
@@ -143,8 +159,13 @@ extern "C" tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_pe
     // entry_src: transpose CSR index -> forward CSR index -> forward tiled index
     std::vector<int32_t> fwd_tiled_of_csr(nnz);
     for (int e = 0; e < nnz; e++) fwd_tiled_of_csr[fwd->host.entry_src[e]] = e;
-    for (int e = 0; e < nnz; e++) p->host.entry_src[e] = fwd_tiled_of_csr[perm[p->host.entry_src[e]]];
+    p->entry_csr.resize(nnz);
+    for (int e = 0; e < nnz; e++) {
+        p->entry_csr[e] = perm[p->host.entry_src[e]];
+        p->host.entry_src[e] = fwd_tiled_of_csr[p->entry_csr[e]];
+    }
     p->transposed = 1;
+    p->pattern_key = fwd->pattern_key;
     p->csr_row_offset = std::move(rpT);
     p->csr_col_idx = std::move(cT);
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -75,6 +75,12 @@ struct tca_plan {
     std::vector<UnitTable> unit_tables;
     // CSR pattern kept for building the transpose
     std::vector<int32_t> csr_row_offset, csr_col_idx;
+    // transposed plans: CSR index (in the forward matrix) of each entry; forward
+    // plans use host.entry_src
+    std::vector<int32_t> entry_csr;
+    // digest of the forward matrix's pattern (shape, row_offset, col_idx), the
+    // same for a forward plan and every transpose made from it
+    uint64_t pattern_key = 0;
 };
 
 namespace tca {
@@ -148,11 +154,26 @@ struct ConvDesc {              // an FP32 operand a batch rounds to an FP16 copy
     int32_t rows, N, ld, pad;
     int64_t first8;            // its first 8-element group in the conversion launch (a multiple of 256)
 };
-// Rounds every operand of descs (nearest even, the rounding the kernels apply
-// on load) into its FP16 copy: one launch over total8 groups of 8 elements,
-// at most kMaxConv operands.
+// FP16 copies: every operand of a ConvDesc list is rounded (nearest even, the
+// rounding the kernels apply on load) into its copy, total8 groups of 8
+// elements over all of them, at most kMaxConv operands (launch_prologue).
 constexpr int kMaxConv = 256;
-tca_status launch_convert_f16(const ConvDesc *descs, int n, long total8, hipStream_t stream);
+struct PackDesc {              // panel-dense tiles a batch rebuilds from weight values on every run
+    const int32_t *seg_ptr;    // the plan's (chunk, panel) entry ranges: tile t = seg_ptr[t]..seg_ptr[t+1]
+    const uint16_t *entry_slot;
+    const int32_t *map;        // index of each entry's value in src (null: the entry's own index)
+    const uint16_t *src;       // weight values (16-bit)
+    uint16_t *dst;             // n_tiles x 512 elements
+    int64_t n_tiles;
+    int64_t first_block;       // its first workgroup in the prologue's pack range
+};
+constexpr int kMaxPack = 512;
+constexpr int kPackTilesPerBlock = 4;
+// One launch: every FP16 copy (conversion groups of 8, blocks [0, conv_blocks))
+// and every dense-tile pack (blocks from conv_blocks on, kPackTilesPerBlock
+// tiles each).  total8 as for the conversion; pack_blocks = the packs' blocks.
+tca_status launch_prologue(const ConvDesc *conv, int n_conv, long total8, const PackDesc *pack, int n_pack,
+                           long pack_blocks, hipStream_t stream);
 constexpr int kBatchSddmmCB = 4;   // chunks per SDDMM unit in a batch
 // Work counters of a batch launch: the unit list is cut into up to
 // kBatchParts partitions (one per XCD under round-robin workgroup dispatch),

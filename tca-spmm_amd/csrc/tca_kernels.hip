@@ -1682,40 +1682,79 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
 
 namespace {
 
-// One thread per 8 consecutive elements of one row: two 16-byte loads, one
-// 16-byte store.  Every operand starts on a 256-group boundary (first8), so a
-// workgroup's operand is uniform; the operands' first groups are staged in LDS
-// once (one load per thread, at most kMaxConv operands) and scanned there.
-__global__ __launch_bounds__(256) void convert_f16_kernel(const ConvDesc *descs, int n, long total8)
+// Prologue of a batch run, one launch.  Blocks [0, conv_blocks): FP32 -> FP16
+// copies, one thread per 8 consecutive elements of one row (two 16-byte loads,
+// one 16-byte store); every operand starts on a 256-group boundary (first8),
+// so a workgroup's operand is uniform.  Blocks from conv_blocks on: panel-dense
+// tiles, one wave per tile as in pack_dense_kernel, the values read through the
+// item's map (weight order -> the plan's entries).  Each block finds its
+// descriptor by a binary search over the first groups / blocks staged in LDS.
+__device__ __forceinline__ int find_first(const long *first, int n, long x)
 {
-    __shared__ long first[kMaxConv];
-    if ((int)threadIdx.x < n) first[threadIdx.x] = gbl(descs)[threadIdx.x].first8;
-    __syncthreads();
-    for (long blk = blockIdx.x; blk * 256 < total8; blk += gridDim.x) {
+    int lo = 0, hi = n - 1;   // last k with first[k] <= x (first[0] == 0)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void prologue_kernel(const ConvDesc *conv, int n_conv, long conv_blocks,
+                                                       const PackDesc *pack, int n_pack)
+{
+    __shared__ long first[kMaxConv > kMaxPack ? kMaxConv : kMaxPack];
+    __shared__ __attribute__((aligned(16))) uint16_t tile[kPackTilesPerBlock][512];
+    const long blk = blockIdx.x;
+    if (blk < conv_blocks) {
+        for (int i = threadIdx.x; i < n_conv; i += 256) first[i] = conv[i].first8;
+        __syncthreads();
         const long t0 = blk * 256;
-        int k = 0;
-        while (k + 1 < n && first[k + 1] <= t0) k++;
-        const float *src0 = descs[k].src;
-        uint16_t *dst0 = descs[k].dst;
-        const int N = descs[k].N, ld = descs[k].ld;
-        const long e = (t0 + threadIdx.x - descs[k].first8) * 8;
-        if (e >= (long)descs[k].rows * N) continue;   // this operand's padding
-        const long r = e / N, col = e - r * N;
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(src0 + r * ld + col);
+        const int k = find_first(first, n_conv, t0);
+        const ConvDesc d = conv[k];
+        const long e = (t0 + threadIdx.x - d.first8) * 8;
+        if (e >= (long)d.rows * d.N) return;   // this operand's padding
+        const long r = e / d.N, col = e - r * d.N;
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(d.src + r * d.ld + col);
         const u32x4 a = gbl(src)[0], b = gbl(src)[1];
         auto h = [](uint32_t x) { return f32_to_f16_bits(__builtin_bit_cast(float, x)); };
         const u32x4 o = {h(a[0]) | h(a[1]) << 16, h(a[2]) | h(a[3]) << 16, h(b[0]) | h(b[1]) << 16, h(b[2]) | h(b[3]) << 16};
-        gbl(reinterpret_cast<u32x4 *>(dst0 + r * N + col))[0] = o;
+        gbl(reinterpret_cast<u32x4 *>(d.dst + r * d.N + col))[0] = o;
+        return;
     }
+    for (int i = threadIdx.x; i < n_pack; i += 256) first[i] = pack[i].first_block;
+    __syncthreads();
+    const long pb = blk - conv_blocks;
+    const int k = find_first(first, n_pack, pb);
+    const PackDesc d = pack[k];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long t = (pb - d.first_block) * kPackTilesPerBlock + w;
+    if (t >= d.n_tiles) return;   // whole waves; no workgroup barrier below
+    uint16_t *tl = tile[w];
+    reinterpret_cast<uint4 *>(tl)[lane] = make_uint4(0, 0, 0, 0);
+    wave_lds_sync();
+    const int e1 = gbl(d.seg_ptr)[t + 1];
+    for (int e = gbl(d.seg_ptr)[t] + lane; e < e1; e += 64) {
+        const int slot = gbl(d.entry_slot)[e], r = slot >> 5, kk = slot & 31;
+        tl[((kk >> 3) * 16 + r) * 8 + (kk & 7)] = gbl(d.src)[d.map ? gbl(d.map)[e] : e];
+    }
+    wave_lds_sync();
+    gbl(reinterpret_cast<u32x4 *>(d.dst + t * 512))[lane] = reinterpret_cast<const u32x4 *>(tl)[lane];
 }
 
 }  // namespace
 
-tca_status launch_convert_f16(const ConvDesc *descs, int n, long total8, hipStream_t stream)
+tca_status launch_prologue(const ConvDesc *conv, int n_conv, long total8, const PackDesc *pack, int n_pack,
+                           long pack_blocks, hipStream_t stream)
 {
-    if (n <= 0 || total8 <= 0) return TCA_OK;
-    const long blocks = std::min<long>((total8 + 255) / 256, 8192);
-    hipLaunchKernelGGL(convert_f16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, descs, n, total8);
+    if (n_conv > kMaxConv || n_pack > kMaxPack) return fail(TCA_ERR_INVALID_ARG, "batch prologue: too many operands");
+    const long conv_blocks = n_conv > 0 ? (total8 + 255) / 256 : 0;
+    if (n_pack <= 0) pack_blocks = 0;
+    const long blocks = conv_blocks + pack_blocks;
+    if (blocks <= 0) return TCA_OK;
+    if (blocks > 0x7fffffffL) return fail(TCA_ERR_UNSUPPORTED, "batch prologue: grid too large");
+    hipLaunchKernelGGL(prologue_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, conv, n_conv, conv_blocks, pack,
+                       n_pack);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }

@@ -38,7 +38,8 @@ class BatchItem(ctypes.Structure):
     _fields_ = [("op", ctypes.c_int), ("plan", ctypes.c_void_p), ("tile_n", ctypes.c_int),
                 ("a_dtype", ctypes.c_int), ("a", ctypes.c_void_p), ("lda", ctypes.c_int),
                 ("b_dtype", ctypes.c_int), ("b", ctypes.c_void_p), ("ldb", ctypes.c_int),
-                ("N", ctypes.c_int), ("out", ctypes.c_void_p), ("ldc", ctypes.c_int), ("a_dense", ctypes.c_int)]
+                ("N", ctypes.c_int), ("out", ctypes.c_void_p), ("ldc", ctypes.c_int), ("a_dense", ctypes.c_int),
+                ("values_order", ctypes.c_void_p)]
 
 
 _vp = ctypes.c_void_p
@@ -92,6 +93,7 @@ SIGNATURES = {
     "tca_quantize_cols_e4m3": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _vp]),
     "tca_batch_create": (_i, [_i, ctypes.POINTER(BatchItem), ctypes.POINTER(_vp)]),
     "tca_batch_run": (_i, [_vp, _vp]),
+    "tca_batch_run_stage": (_i, [_vp, _i, _vp]),
     "tca_batch_destroy": (_i, [_vp]),
     "tca_batch_get_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tca_rocsparse_spmm": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _dp]),

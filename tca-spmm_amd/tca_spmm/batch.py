@@ -7,6 +7,13 @@
 Items are fixed at construction (pointers, shapes, tile widths); run() only
 launches.  Calls that overlap on one device must use different Batch objects
 (each owns its work counter).
+
+Sparse-weight training: spmm_item(plan, w, X, Y, dense="pack", values_order=p)
+hands the batch the weight VALUES (in p's tiled order, or CSR order when
+values_order is None); the batch owns the item's panel-dense tiles and rebuilds
+them from w at the start of every run, so an optimizer that updates w in place
+needs no repack launches of its own: run() = one prologue launch (FP16 copies
+of FP32 operands, dense tiles) + the persistent launch.
 """
 import ctypes
 
@@ -17,9 +24,17 @@ from ._lib import OP_SDDMM, OP_SPMM, call
 from .core import _stream, tca_dtype
 
 
-def spmm_item(plan, a_tiled, B, out, tile_n=0, dense=False):
-    """dense=True: a_tiled holds the plan's panel-dense tiles (Plan.pack_dense)."""
-    return dict(op=OP_SPMM, plan=plan, a=a_tiled, b=B, out=out, tile_n=tile_n, dense=dense)
+def spmm_item(plan, a_tiled, B, out, tile_n=0, dense=False, values_order=None):
+    """dense=True: a_tiled holds the plan's panel-dense tiles (Plan.pack_dense).
+    dense="pack": a_tiled holds weight values in values_order's tiled order
+    (a forward Plan of the same pattern; None = CSR order) and the batch packs
+    the plan's dense tiles from them on every run."""
+    if dense not in (False, True, "pack"):
+        raise ValueError("dense must be False, True or 'pack'")
+    if values_order is not None and dense != "pack":
+        raise ValueError("values_order applies to dense='pack' items")
+    return dict(op=OP_SPMM, plan=plan, a=a_tiled, b=B, out=out, tile_n=tile_n, dense=dense,
+                values_order=values_order)
 
 
 def sddmm_item(plan, G, X, out):
@@ -46,7 +61,12 @@ class Batch:
             e.N = b.shape[1]
             e.out = out.data_ptr()
             e.ldc = out.stride(0) if out.dim() == 2 else 0
-            e.a_dense = 1 if it.get("dense") else 0
+            d = it.get("dense", False)
+            e.a_dense = 2 if d == "pack" else (1 if d else 0)
+            vo = it.get("values_order")
+            if vo is not None:
+                self._keep.append(vo)
+                e.values_order = vo.handle
         h = ctypes.c_void_p()
         call("tca_batch_create", len(items), arr, ctypes.byref(h))
         self._h = h
@@ -54,8 +74,13 @@ class Batch:
         call("tca_batch_get_info", h, ctypes.byref(nu), ctypes.byref(nw))
         self.n_units, self.n_workgroups = nu.value, nw.value
 
-    def run(self, stream=None):
-        call("tca_batch_run", self._h, _stream(stream))
+    def run(self, stream=None, stage=3):
+        """stage 1: the prologue only (FP16 copies, dense tiles), 2: the
+        persistent launch only, 3: both (a full run)."""
+        if stage == 3:
+            call("tca_batch_run", self._h, _stream(stream))
+        else:
+            call("tca_batch_run_stage", self._h, stage, _stream(stream))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

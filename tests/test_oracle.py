@@ -233,6 +233,29 @@ def test_sddmm_f64_against_dense():
     assert not np.allclose(out, (X @ G.T)[ci % M, rows % K]) if M != K else True
 
 
+def test_sddmm_f32_exact_on_small_integers_and_close_to_f64():
+    # small-integer fp16 operands: every product and partial sum is exact in
+    # FP32, so the result must equal numpy's dense product sampled on the
+    # pattern exactly; a transposed operand or a dropped n would not
+    M, K, N = 23, 37, 96
+    dense, rp, ci, v = _rand_csr(M, K, 0.3, 12)
+    rng = np.random.default_rng(13)
+    G = rng.integers(-8, 9, (M, N)).astype(np.float16)
+    X = rng.integers(-8, 9, (K, N)).astype(np.float16)
+    out = O.sddmm_f32(M, N, rp, ci, G.view(np.uint16), X.view(np.uint16))
+    rows = np.repeat(np.arange(M), np.diff(rp))
+    full = G.astype(np.float64) @ X.astype(np.float64).T
+    assert np.array_equal(out.astype(np.float64), full[rows, ci])
+    short = G.astype(np.float64)[:, :-1] @ X.astype(np.float64)[:, :-1].T
+    assert not np.array_equal(out.astype(np.float64), short[rows, ci])
+    # random normal operands: FP32 accumulation stays within 1e-6 of FP64
+    Gn = rng.standard_normal((M, N)).astype(np.float16)
+    Xn = rng.standard_normal((K, N)).astype(np.float16)
+    o32 = O.sddmm_f32(M, N, rp, ci, Gn.view(np.uint16), Xn.view(np.uint16))
+    o64 = O.sddmm_f64(M, N, rp, ci, Gn.astype(np.float64), Xn.astype(np.float64))
+    assert np.linalg.norm(o32 - o64) / np.linalg.norm(o64) < 1e-6
+
+
 # ------------------------------------------------------ workload_manager --
 def test_workload_manager_known_answer_bundled():
     rc, out = O.smtx_to_dense(BUNDLED_SMTX)
