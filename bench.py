@@ -109,9 +109,20 @@ def rel_l2(np, got, ref):
 # the twelve-layer training step
 # --------------------------------------------------------------------------
 class TrainStep:
-    """Weights of all layers in one FP16 buffer (each in its 16-row plan's
-    tiled order, the order SDDMM writes dW in), dW of all layers in one FP32
-    buffer, so the SGD update is one launch.  Three ways to run the step."""
+    """The step run three ways, each path with the weights in its own
+    preferred order, its own FP16 weight buffer and FP32 dW buffer (all layers
+    in one buffer each, so its SGD update is one launch):
+
+      "f"  fused: weights in the tiled order of the batch's forward plan (the
+           order its SDDMM writes dW in); the batch packs the forward and
+           transposed dense tiles from them in its prologue launch;
+      "t"  plain per call: weights in the tiled order of each layer's tuned
+           forward plan; forward, dX (transposed plan reading the same values)
+           and dW are 36 tca_spmm / tca_sddmm launches, no repack needed;
+      "d"  per call on panel-dense tiles: as "t" with the tuned dense plans,
+           plus the repack of both dense tile sets after every update."""
+
+    PATHS = ("f", "t", "d")
 
     def __init__(self, torch, np, T, dev, seed=0):
         from tca_spmm import workloads
@@ -122,42 +133,44 @@ class TrainStep:
         for c in cases:
             offs.append(tot)
             tot += (c.A.nnz + 7) // 8 * 8            # 16-byte aligned views
-        self.w_all = torch.zeros(max(tot, 8), dtype=torch.float16, device=dev)
-        self.dw_all = torch.zeros(max(tot, 8), dtype=torch.float32, device=dev)
+        self.w_all = {p: torch.zeros(max(tot, 8), dtype=torch.float16, device=dev) for p in self.PATHS}
+        self.dw_all = {p: torch.zeros(max(tot, 8), dtype=torch.float32, device=dev) for p in self.PATHS}
         self.layers, self.flops, self.alg_bytes, self.dense_flops = [], 0, 0, 0
         items = []
         for c, off in zip(cases, offs):
             W, N = c.A, c.N
-            L = dict(case=c, N=N)
-            p16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
-            v_csr = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
-            w = self.w_all[off:off + W.nnz]
-            w.copy_(p16.pack(v_csr))
-            L.update(p16=p16, w=w, dW=self.dw_all[off:off + W.nnz],
+            L = dict(case=c, N=N,
                      X=torch.from_numpy(c.B.view(np.int16)).view(torch.float16).to(dev),
                      G=torch.from_numpy(c.extra["G"]).to(dev),
                      Y=torch.empty((W.n_rows, N), dtype=torch.float32, device=dev),
                      dX=torch.empty((W.n_cols, N), dtype=torch.float32, device=dev))
-            # fused: the batch packs dense tiles of the forward and transposed plans
-            rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)
-            rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True)
-            pf = p16 if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
-            pt = p16.transpose(rpw_t)
-            L["fused"] = dict(fwd=(rpw, tile), dx=(rpw_t, tile_t))
-            items += [T.spmm_item(pf, w, L["X"], L["Y"], tile, dense="pack", values_order=p16),
-                      T.spmm_item(pt, w, L["G"], L["dX"], tile_t, dense="pack", values_order=p16),
-                      T.sddmm_item(p16, L["G"], L["X"], L["dW"])]
-            # per call: each product's tuned launch, tiled values ("t") and dense tiles ("d")
-            for lay, dense in (("t", False), ("d", True)):
-                r, t = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, dense=dense)
-                r_t, t_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", dense=dense)
-                plan = p16 if r == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, r)
-                plan_t = p16.transpose(r_t)
-                own = None if plan is p16 else torch.empty_like(w)          # plan's own tiled values
-                L[lay] = dict(plan=plan, plan_t=plan_t, tile=t, tile_t=t_t, own=own, choice=[[r, t], [r_t, t_t]],
-                              a=(plan.pack_dense(w if own is None else own) if dense else None),
-                              at=(plan_t.pack_dense(w) if dense else None))
-            L["csr"] = torch.empty_like(w)
+            v_csr = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
+            plans, tplans = {}, {}
+            for path in self.PATHS:
+                if path == "f":
+                    r, t = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)
+                    r_t, t_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True)
+                else:
+                    dense = path == "d"
+                    r, t = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, dense=dense)
+                    r_t, t_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", dense=dense)
+                if r not in plans:
+                    plans[r] = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, r)
+                plan = plans[r]
+                if (r, r_t) not in tplans:
+                    tplans[(r, r_t)] = plan.transpose(r_t)
+                plan_t = tplans[(r, r_t)]
+                w = self.w_all[path][off:off + W.nnz]
+                w.copy_(plan.pack(v_csr))
+                P = dict(plan=plan, plan_t=plan_t, tile=t, tile_t=t_t, w=w, dW=self.dw_all[path][off:off + W.nnz],
+                         choice=[[r, t], [r_t, t_t]])
+                if path == "d":
+                    P["a"], P["at"] = plan.pack_dense(w), plan_t.pack_dense(w)
+                L[path] = P
+            f = L["f"]
+            items += [T.spmm_item(f["plan"], f["w"], L["X"], L["Y"], f["tile"], dense="pack", values_order=f["plan"]),
+                      T.spmm_item(f["plan_t"], f["w"], L["G"], L["dX"], f["tile_t"], dense="pack", values_order=f["plan"]),
+                      T.sddmm_item(f["plan"], L["G"], L["X"], f["dW"])]
             self.layers.append(L)
             self.flops += 3 * 2 * W.nnz * N
             self.dense_flops += 3 * 2 * W.n_rows * W.n_cols * N
@@ -165,58 +178,54 @@ class TrainStep:
                                + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, N))
         self.batch = T.Batch(items)
 
-    def sgd(self):
-        self.w_all.add_(self.dw_all, alpha=-SGD_LR)
+    def sgd(self, path):
+        self.T.sgd_update(self.w_all[path], self.dw_all[path], SGD_LR)
 
-    def repack(self, lay):
-        """What the per-call path `lay` rebuilds after a weight update: its own
-        tiled values of plans other than the 16-row one, and its dense tiles."""
-        for L in self.layers:
-            c = L[lay]
-            if c["own"] is not None:
-                L["p16"].unpack(L["w"], out=L["csr"])
-                c["plan"].pack(L["csr"], out=c["own"])
-            if lay == "d":
-                c["plan"].pack_dense(L["w"] if c["own"] is None else c["own"], out=c["a"])
-                c["plan_t"].pack_dense(L["w"], out=c["at"])
+    def repack(self, path):
+        """What a path rebuilds after a weight update: the dense tiles of "d"
+        (the fused path packs inside its batch run; "t" reads its values)."""
+        if path == "d":
+            for L in self.layers:
+                P = L["d"]
+                P["plan"].pack_dense(P["w"], out=P["a"])
+                P["plan_t"].pack_dense(P["w"], out=P["at"])
 
-    def products(self, lay, op=None):
+    def products(self, path, op=None):
         T = self.T
+        dense = path == "d"
         for L in self.layers:
-            c = L[lay]
-            dense = lay == "d"
-            a = c["a"] if dense else (L["w"] if c["own"] is None else c["own"])
+            P = L[path]
             if op in (None, "fwd"):
-                T.spmm(c["plan"], a, L["X"], out=L["Y"], tile_n=c["tile"], dense=dense)
+                T.spmm(P["plan"], P["a"] if dense else P["w"], L["X"], out=L["Y"], tile_n=P["tile"], dense=dense)
             if op in (None, "dx"):
-                T.spmm(c["plan_t"], c["at"] if dense else L["w"], L["G"], out=L["dX"], tile_n=c["tile_t"], dense=dense)
+                T.spmm(P["plan_t"], P["at"] if dense else P["w"], L["G"], out=L["dX"], tile_n=P["tile_t"], dense=dense)
             if op in (None, "dw"):
-                T.sddmm(L["p16"], L["G"], L["X"], out=L["dW"])
+                T.sddmm(P["plan"], L["G"], L["X"], out=P["dW"])
 
     def fused_step(self):
-        self.sgd()
+        self.sgd("f")
         self.batch.run()
 
-    def per_call_step(self, lay):
+    def per_call_step(self, path):
         def run():
-            self.sgd()
-            self.repack(lay)
-            self.products(lay)
+            self.sgd(path)
+            self.repack(path)
+            self.products(path)
         return run
 
     def parity_gate(self, O):
         """Fused outputs of every layer against the per-call tiled launches
-        (FP32 summation order) and two layers against the FP64 oracle."""
+        (FP32 summation order; dW mapped through CSR order) and two layers
+        against the FP64 oracle."""
         torch, np = self.torch, self.np
         self.batch.run()
         torch.cuda.synchronize()
-        fused = [(L["Y"].clone(), L["dX"].clone(), L["dW"].clone()) for L in self.layers]
-        self.repack("t")
+        fused = [(L["Y"].clone(), L["dX"].clone(), L["f"]["plan"].unpack(L["f"]["dW"])) for L in self.layers]
         self.products("t")
         torch.cuda.synchronize()
         worst = 0.0
         for L, (y, dx, dw) in zip(self.layers, fused):
-            for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["dW"])):
+            for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["t"]["plan"].unpack(L["t"]["dW"]))):
                 r = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
                 worst = max(worst, r)
                 if r > 1e-6:
@@ -225,7 +234,7 @@ class TrainStep:
         for i in (0, len(self.layers) - 1):
             L = self.layers[i]
             W, N = L["case"].A, L["N"]
-            a64 = L["p16"].unpack(L["w"]).cpu().numpy().view(np.float16).astype(np.float64)
+            a64 = W.values.view(np.float16).astype(np.float64)
             x64 = L["case"].B.view(np.float16).astype(np.float64)
             g64 = L["case"].extra["G"].astype(np.float16).astype(np.float64)
             ref_y = O.spmm_f64(W.n_rows, N, W.row_offset, W.col_idx, a64, x64)
@@ -234,7 +243,7 @@ class TrainStep:
             ref_dw = O.sddmm_f64(W.n_rows, N, W.row_offset, W.col_idx, g64, x64)
             y, dx, dw = fused[i]
             e = [rel_l2(np, y.cpu().numpy(), ref_y), rel_l2(np, dx.cpu().numpy(), ref_dx),
-                 rel_l2(np, L["p16"].unpack(dw).cpu().numpy(), ref_dw)]
+                 rel_l2(np, dw.cpu().numpy(), ref_dw)]
             if max(e) > 1e-5:
                 raise RuntimeError(f"{W.name}: fused step vs FP64 oracle {e}")
             oracle_err[W.name] = {"fwd": e[0], "dx": e[1], "dw": e[2]}
@@ -283,13 +292,12 @@ def bench_train_step(torch, np, T, O, dev, args):
     ts.batch.run(stage=1)
     main_ms = time_eager(torch, lambda: ts.batch.run(stage=2), K, backlog=True)
     prologue_ms = time_eager(torch, lambda: ts.batch.run(stage=1), K, backlog=True)
-    sgd_ms = time_eager(torch, ts.sgd, K, backlog=True)
+    sgd_ms = time_eager(torch, lambda: ts.sgd("f"), K, backlog=True)
     graph_ms = time_graph(torch, ts.fused_step, K)
     # the per-call paths, each with its own repack and the same SGD update
     plain_graph = time_graph(torch, ts.per_call_step("t"), 20)
     dense_graph = time_graph(torch, ts.per_call_step("d"), 20)
     plain_eager = time_eager(torch, ts.per_call_step("t"), 10)
-    repack_t = time_graph(torch, lambda: ts.repack("t"), 20)
     repack_d = time_graph(torch, lambda: ts.repack("d"), 20)
     per_op = {lay: {op: time_graph(torch, (lambda lay=lay, op=op: ts.products(lay, op)), 20)
                     for op in ("fwd", "dx", "dw")} for lay in ("t", "d")}
@@ -298,14 +306,14 @@ def bench_train_step(torch, np, T, O, dev, args):
         "dense_equivalent_flops_per_step": ts.dense_flops, "alg_bytes_per_step": ts.alg_bytes,
         "main_launch_ms": main_ms, "prologue_ms": prologue_ms, "sgd_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
         "plain_per_call_graph_ms": plain_graph, "plain_per_call_eager_ms": plain_eager,
-        "dense_per_call_graph_ms": dense_graph, "plain_repack_ms": repack_t, "dense_repack_ms": repack_d,
+        "dense_per_call_graph_ms": dense_graph, "dense_repack_ms": repack_d,
         "speedup_vs_plain_per_call": plain_graph / ms_per_step,
         "speedup_vs_dense_per_call": dense_graph / ms_per_step,
         "speedup_vs_plain_per_call_eager": plain_eager / ms_per_step,
         "per_call_graph_ms_by_op": {"tiled": per_op["t"], "dense": per_op["d"]},
         "units": ts.batch.n_units, "workgroups": ts.batch.n_workgroups,
         "parity": {"worst_rel_l2_vs_per_call": worst, "rel_l2_vs_fp64_oracle": oracle_err},
-        "layers": [{"name": L["case"].A.name, "nnz": L["case"].A.nnz, "N": L["N"], "fused": L["fused"],
+        "layers": [{"name": L["case"].A.name, "nnz": L["case"].A.nnz, "N": L["N"], "fused": L["f"]["choice"],
                     "per_call_tiled": L["t"]["choice"], "per_call_dense": L["d"]["choice"]} for L in ts.layers],
     }
     if not args.no_libs:
@@ -315,7 +323,7 @@ def bench_train_step(torch, np, T, O, dev, args):
                 W = L["case"].A
                 ro = torch.from_numpy(W.row_offset).to(dev)
                 ci = torch.from_numpy(W.col_idx).to(dev)
-                vals = L["p16"].unpack(L["w"])
+                vals = L["t"]["plan"].unpack(L["t"]["w"])
                 g16 = L["G"].half()
                 out = torch.empty((W.n_cols, L["N"]), dtype=torch.float32, device=dev)
                 lib_dx += T.rocsparse_spmm_t(W.n_rows, L["N"], W.n_cols, ro, ci, vals, g16, out, n_iter=5)

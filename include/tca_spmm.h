@@ -264,6 +264,13 @@ tca_status tca_quantize_rows_e4m3(int n_rows, const int32_t *d_row_offset, const
 tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, uint8_t *d_q, int ldq,
                                   float *d_col_scale, void *stream);
 
+/* Plain SGD step on weight values held in any order (tiled or CSR):
+ * w[i] = round_to_nearest_even(w[i] - lr * g[i]) for i < n, w F16 or BF16, g FP32
+ * in the same order (e.g. the dW a batch's SDDMM item writes).  w and g 16-byte
+ * aligned.  No counterpart in the reference (it has no training); the
+ * sparse-weight training step uses it so that the update is one launch. */
+tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t n, float lr, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched persistent launch over many layers (one kernel, capturable).      */
 /* SPMM items: F16 values; forward with an F16 B on 16-row plans (tile_n     */

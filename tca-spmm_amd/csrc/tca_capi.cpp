@@ -465,3 +465,12 @@ extern "C" tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, 
     if ((K + 63) / 64 > 65535) return fail(TCA_ERR_UNSUPPORTED, "tca_quantize_cols_e4m3: K above 4194240");
     return tca::launch_quantize_cols(K, N, d_B, ldb, d_q, ldq, d_col_scale, static_cast<hipStream_t>(stream));
 }
+
+extern "C" tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t n, float lr, void *stream)
+{
+    if (w_dtype != TCA_F16 && w_dtype != TCA_BF16) return fail(TCA_ERR_UNSUPPORTED, "tca_sgd_update: weights must be F16 or BF16");
+    if (n == 0) return TCA_OK;
+    if (!d_w || !d_g) return fail(TCA_ERR_INVALID_ARG, "tca_sgd_update: NULL buffer");
+    if (!aligned(d_w, 16) || !aligned(d_g, 16)) return fail(TCA_ERR_INVALID_ARG, "tca_sgd_update: w and g must be 16-byte aligned");
+    return tca::launch_sgd(w_dtype == TCA_BF16, d_w, d_g, (long)n, lr, static_cast<hipStream_t>(stream));
+}

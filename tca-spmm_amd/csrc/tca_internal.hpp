@@ -97,6 +97,7 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
                         const void *X, int ldx, int N, float *out, hipStream_t stream);
 tca_status launch_gather(const int32_t *idx, int n, int elem_bytes, const void *src, void *dst, bool scatter,
                          hipStream_t stream);
+tca_status launch_sgd(bool bf16, void *w, const float *g, long n, float lr, hipStream_t stream);
 tca_status launch_quantize_rows(int n_rows, const int32_t *row_offset, const uint16_t *vals, uint8_t *q,
                                 float *scale, hipStream_t stream);
 tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq, float *scale,

@@ -1100,6 +1100,33 @@ __global__ void gather_kernel(const int32_t *__restrict__ idx, int n, const T *_
     }
 }
 
+// w -= lr * g, 8 elements per thread (one 16-byte load of w, two of g, one
+// 16-byte store); the last n % 8 elements one per thread of the first block.
+template <bool BF>
+__global__ __launch_bounds__(256) void sgd_kernel(uint16_t *__restrict__ w, const float *__restrict__ g, long n8,
+                                                  long n, float lr)
+{
+    auto upd = [&](uint32_t bits, float gv) -> uint32_t {
+        const float x = BF ? __builtin_bit_cast(float, bits << 16) : (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
+        const float y = fmaf(-lr, gv, x);
+        return BF ? f32_to_bf16_bits(y) : f32_to_f16_bits(y);
+    };
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (long)gridDim.x * blockDim.x) {
+        const u32x4 wv = reinterpret_cast<const u32x4 *>(w)[i];
+        const float4 g0 = reinterpret_cast<const float4 *>(g)[2 * i], g1 = reinterpret_cast<const float4 *>(g)[2 * i + 1];
+        u32x4 o;
+        o[0] = upd(wv[0] & 0xffffu, g0.x) | upd(wv[0] >> 16, g0.y) << 16;
+        o[1] = upd(wv[1] & 0xffffu, g0.z) | upd(wv[1] >> 16, g0.w) << 16;
+        o[2] = upd(wv[2] & 0xffffu, g1.x) | upd(wv[2] >> 16, g1.y) << 16;
+        o[3] = upd(wv[3] & 0xffffu, g1.z) | upd(wv[3] >> 16, g1.w) << 16;
+        reinterpret_cast<u32x4 *>(w)[i] = o;
+    }
+    if (blockIdx.x == 0) {
+        const long t = 8 * n8 + threadIdx.x;
+        if (t < n) w[t] = (uint16_t)upd(w[t], g[t]);
+    }
+}
+
 // C[0:rows, 0:N] = 0 (before split-chunk launches accumulate into it)
 __global__ void zero_rows_kernel(float *C, int rows, int N, int ldc)
 {
@@ -1799,6 +1826,21 @@ tca_status launch_pack_dense(const tca_plan *p, int elem_bytes, const void *tile
                            p->d_entry_slot, src_idx, static_cast<const uint8_t *>(tiled), static_cast<uint8_t *>(dense));
     else
         return fail(TCA_ERR_INVALID_ARG, "tca_pack_dense: element size");
+    TCA_HIP_TRY(hipGetLastError());
+    return TCA_OK;
+}
+
+tca_status launch_sgd(bool bf16, void *w, const float *g, long n, float lr, hipStream_t stream)
+{
+    if (n <= 0) return TCA_OK;
+    const long n8 = n / 8;
+    const long blocks = std::max<long>(1, std::min<long>((n8 + 255) / 256, 8192));
+    if (bf16)
+        hipLaunchKernelGGL(sgd_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, stream, static_cast<uint16_t *>(w), g,
+                           n8, n, lr);
+    else
+        hipLaunchKernelGGL(sgd_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, stream, static_cast<uint16_t *>(w), g,
+                           n8, n, lr);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
 }

@@ -13,7 +13,7 @@ lib()  # fail loudly at import when the native library is absent
 
 try:  # torch-facing API (torch is the device-memory layer, not the compute)
     from .core import (Plan, PlanCache, csr_transpose_device, hipblaslt_gemm, rocsparse_sddmm, rocsparse_spmm_t,  # noqa: F401
-                       quantize_cols_e4m3, quantize_rows_e4m3, rocsparse_spmm, sddmm, spmm)
+                       quantize_cols_e4m3, quantize_rows_e4m3, rocsparse_spmm, sddmm, sgd_update, spmm)
     from .autograd import SparseLinear, SpmmFunction, sparse_matmul  # noqa: F401
     from .batch import Batch, sddmm_item, spmm_item  # noqa: F401
     from .bell import BlockedEll, bell_spmm  # noqa: F401
@@ -21,5 +21,5 @@ try:  # torch-facing API (torch is the device-memory layer, not the compute)
 except ImportError:  # pragma: no cover - torch absent
     pass
 
-__all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
+__all__ = ["Plan", "PlanCache", "csr_transpose_device", "spmm", "sddmm", "sgd_update", "quantize_rows_e4m3", "quantize_cols_e4m3", "rocsparse_spmm",
            "hipblaslt_gemm", "rocsparse_sddmm", "rocsparse_spmm_t", "hipsparselt_spmm_24", "SparseLinear", "sparse_matmul", "Batch", "spmm_item", "sddmm_item", "gen", "workloads", "bell", "bell_from_csr", "bell_shape", "BlockedEll", "bell_spmm", "Sparse24", "spmm_24", "TcaError", "F16", "BF16", "E4M3", "F32"]

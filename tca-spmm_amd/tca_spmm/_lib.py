@@ -92,6 +92,7 @@ SIGNATURES = {
     "tca_quantize_rows_e4m3": (_i, [_i, _vp, _vp, _vp, _vp, _vp]),
     "tca_quantize_cols_e4m3": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _vp]),
     "tca_batch_create": (_i, [_i, ctypes.POINTER(BatchItem), ctypes.POINTER(_vp)]),
+    "tca_sgd_update": (_i, [_i, _vp, _vp, ctypes.c_size_t, ctypes.c_float, _vp]),
     "tca_batch_run": (_i, [_vp, _vp]),
     "tca_batch_run_stage": (_i, [_vp, _i, _vp]),
     "tca_batch_destroy": (_i, [_vp]),

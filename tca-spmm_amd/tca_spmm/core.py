@@ -239,6 +239,17 @@ def sddmm(plan, G, X, out=None, stream=None):
     return out
 
 
+def sgd_update(w, g, lr, stream=None):
+    """w -= lr * g in place (w float16/bfloat16, g float32, same length and
+    order, e.g. tiled weight values and the dW an SDDMM wrote)."""
+    if w.numel() != g.numel() or g.dtype != torch.float32 or not (w.is_contiguous() and g.is_contiguous()):
+        raise ValueError("sgd_update: w and g must be contiguous, of equal length, g float32")
+    if w.device != g.device:
+        raise ValueError("sgd_update: w and g must be on one device")
+    call("tca_sgd_update", tca_dtype(w), _ptr(w), _ptr(g), w.numel(), float(lr), _stream(stream))
+    return w
+
+
 def quantize_rows_e4m3(row_offset, values, stream=None):
     """Per-row E4M3 codes (uint8) and FP32 scales of CSR values (float16, device)."""
     n_rows = row_offset.numel() - 1

@@ -168,3 +168,32 @@ def test_train_step_rejects_foreign_values_order(cuda):
     with pytest.raises(T.TcaError) as e:
         T.Batch([T.spmm_item(pa, w, X, Y, 64, dense="pack", values_order=pb)])
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+
+
+@pytest.mark.parametrize("n", [8, 1003, 4096 * 9 + 5])
+def test_sgd_update_matches_fma_rounding(cuda, n):
+    # w <- round(w - lr * g) with one FP32 rounding of the fused multiply-add,
+    # then round to nearest even into the weight type; the tail (n % 8) included
+    rng = np.random.default_rng(n)
+    lr = np.float32(0.0375)
+    g = rng.standard_normal(n).astype(np.float32)
+    for wt, npt in ((torch.float16, np.float16), (torch.bfloat16, None)):
+        w32 = rng.standard_normal(n).astype(np.float32)
+        if npt is not None:
+            w_host = w32.astype(np.float16)
+            w = torch.from_numpy(w_host.view(np.int16)).view(torch.float16).to(cuda)
+            x = w_host.astype(np.float64)
+        else:
+            w = torch.from_numpy(w32).to(cuda).to(torch.bfloat16)
+            x = w.float().cpu().numpy().astype(np.float64)
+        want32 = (x - np.float64(lr) * g.astype(np.float64)).astype(np.float32)
+        T.sgd_update(w, dev(g), float(lr))
+        torch.cuda.synchronize()
+        if npt is not None:
+            want = want32.astype(np.float16).view(np.uint16)
+            got = w.cpu().numpy().view(np.uint16)
+        else:
+            want = torch.from_numpy(want32).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+            got = w.view(torch.int16).cpu().numpy().view(np.uint16)
+        diff = np.abs(got.astype(np.int64) - want.astype(np.int64))
+        assert diff.max() <= 1 and (diff == 0).mean() >= 0.9999, (wt, n)
