@@ -142,6 +142,21 @@ tca_status tca_csr_transpose_device(int n_rows, int n_cols, int nnz, const int32
                                     const int32_t *d_col_idx, int32_t *d_col_offset, int32_t *d_row_idx,
                                     int32_t *d_perm, void *stream);
 
+/* tca_csr_transpose_device without any allocation: the scratch (entry rows,
+ * sort keys and values, radix-sort temp; about 16 bytes per entry plus the
+ * sort's temp) lives in one fixed array of the library per GPU, of
+ * tca_csr_transpose_noalloc_max_scratch() bytes (64 MiB, held until the process
+ * ends); patterns that need more are rejected with TCA_ERR_UNSUPPORTED.  Same
+ * arguments and results as tca_csr_transpose_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+tca_status tca_csr_transpose_device_noalloc(int n_rows, int n_cols, int nnz, const int32_t *d_row_offset,
+                                            const int32_t *d_col_idx, int32_t *d_col_offset, int32_t *d_row_idx,
+                                            int32_t *d_perm, void *stream);
+size_t tca_csr_transpose_noalloc_max_scratch(void);
+
 tca_status tca_plan_get_info(const tca_plan *plan, tca_plan_info *info);
 /* Host copies of the tables (for the host-rebuild check).  Sizes:
  * grp_chunk_ptr[n_groups+1], chunk_cols[32*n_chunks], seg_ptr[n_chunks*rpw/16+1],

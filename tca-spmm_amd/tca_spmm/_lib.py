@@ -76,6 +76,8 @@ SIGNATURES = {
     "tca_plan_create_transpose": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
     "tca_plan_destroy": (_i, [_vp]),
     "tca_csr_transpose_device": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tca_csr_transpose_device_noalloc": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tca_csr_transpose_noalloc_max_scratch": (ctypes.c_size_t, []),
     "tca_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
     "tca_plan_export": (_i, [_vp, _i32p, _i32p, _i32p, _u16p, _i32p, _i32p]),
     "tca_pack_values": (_i, [_vp, _i, _vp, _vp, _vp]),
