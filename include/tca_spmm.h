@@ -291,8 +291,9 @@ tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t
 /* SPMM items: F16 values; forward with an F16 B on 16-row plans (tile_n     */
 /* 16/32/64) or 32-row plans (tile_n 32); transposed with an F32 or F16 B on  */
 /* 16-row plans (tile_n 16/32/64); panel-dense F16 A (a_dense), forward or  */
-/* transposed, with an F16 B on 16-row (tile_n 16/32/64), 32-row (32/64) or  */
-/* 64-row (32) plans, or an F32 B on 16-row (16/32/64) or 32-row (32) plans. */
+/* transposed, with an F16 B on 16-row (tile_n 16/32/64), 32-row (32/64),    */
+/* 64-row (32/64) or 128-row (64) plans, or an F32 B on 16-row (16/32/64) or  */
+/* 32-row (32) plans.                                                         */
 /* SDDMM items: F16 X, F32 or F16 G.  Other combinations: UNSUPPORTED.       */
 /* An F32 operand (SpMM B, SDDMM G) gets a batch-owned F16 copy (2 bytes per  */
 /* element) that every run refreshes first -- same bits as rounding on load -- */

@@ -240,8 +240,9 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                                                      " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
                                                      "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 or f16 B "
                                                      "transposed on rows_per_wave 16 with tile_n 16/32/64; panel-dense f16 A "
-                                                     "with f16 B on rows_per_wave 16 (tile_n 16/32/64), 32 (32/64) "
-                                                     "or 64 (32), or f32 B on rows_per_wave 16 (16/32/64) or 32 (32))");
+                                                     "with f16 B on rows_per_wave 16 (tile_n 16/32/64), 32 (32/64), "
+                                                     "64 (32/64) or 128 (64), or f32 B on rows_per_wave 16 (16/32/64) "
+                                                     "or 32 (32))");
             if (it.a_dense && tca::dense_bytes(t, it.a_dtype) >= (1L << 31))
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " dense tiles reach 2 GiB");
             d.ldb = it.ldb;
@@ -257,8 +258,11 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                     units.push_back({i, g, tt, t.grp_chunk_ptr[g], t.grp_chunk_ptr[g + 1], 0, 0, 0});
                     // estimated ns: unit start + per chunk (index + B rows + MFMAs; an
                     // FP32 B gathers twice the bytes; tiled values add the entry work)
+                    // (block units: the four waves share each chunk, about half the
+                    // cost per panel of a wave per 32 rows)
+                    const bool block = d.variant == 22 || d.variant == 23;
                     const double chunk = (100.0 + (it.b_dtype == TCA_F32 ? 280.0 : 150.0) * ppg * tile / 64.0) *
-                                         (it.a_dense ? 1.0 : 1.6);
+                                         (it.a_dense ? 1.0 : 1.6) * (block ? 0.5 : 1.0);
                     cost.push_back(1500.0 + nch * chunk);
                 }
             }

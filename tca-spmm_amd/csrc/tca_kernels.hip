@@ -107,6 +107,8 @@ __device__ inline typename Store<KPL * ES>::T pack_k(const Oct (&rows)[KPL], int
     return out;
 }
 
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
 // Entries prefetched per lane per panel and chunk (covers 128 entries of a
 // 16x32 panel tile; denser tiles take an extra, unprefetched pass).
 constexpr int kEP = 2;
@@ -773,6 +775,200 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block units on panel-dense tiles (plans of 64 / 128 / 256 rows per group).
+// One workgroup of KS x WM waves owns a (group, N-tile of NT columns, chunk
+// range).  The WM waves of a K group share every 32 x NT B chunk tile: it is
+// loaded once per K group (one or two 16-byte pieces per thread), written to
+// LDS row-major with a pitch of NT+16 elements and read back as MFMA B operands
+// with ds_read_b64_tr_b16 (as in the wide kernel), double buffered with one
+// workgroup barrier per chunk.  Wave w of a K group owns panels
+// [w*PPW, (w+1)*PPW) and all NT columns, its A operand octets come straight
+// from the dense tiles.  The KS K groups split the chunk range (each with its
+// own LDS buffers) and hand their partial tiles to group 0 through LDS, panel
+// by panel, so long groups keep several waves per SIMD busy without atomics.
+// Per chunk a K group reads 64*NT bytes of B and PPG KiB of A for PPG*NT/16
+// MFMAs, against 64*NT + 1 KiB per panel for one wave per 16..64 rows.
+// ---------------------------------------------------------------------------
+template <int NT, int WM>
+struct BlockLds {
+    static constexpr int LDB = NT + 16;                  // B row pitch (elements)
+    static constexpr int BTILE = 32 * LDB * 2;           // one chunk's B tile (bytes)
+    static constexpr int SEGS = NT / 8;                  // 16-byte pieces per B row
+    static constexpr int PPT = 32 * SEGS / (WM * 64);    // pieces per thread
+    static_assert(PPT >= 1 && (32 * SEGS) % (WM * 64) == 0, "B pieces must divide over the waves");
+};
+template <int PPW, int NT, int WM, int KS>
+constexpr int block_lds_bytes()
+{
+    return cmax(KS * 2 * BlockLds<NT, WM>::BTILE, (KS - 1) * WM * (NT / 16) * 1024);
+}
+
+template <int PPW, int NT, int WM, int KS, int D, tca_dtype AT, tca_dtype BT>
+__device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int tile, int u_lo, int u_hi, bool atomic,
+                                                uint8_t *smem_all)
+{
+    using L = BlockLds<NT, WM>;
+    using RL = RowLoad<AT, BT>;
+    using RB = RowBuf<AT, BT>;
+    using Oct = uint4;
+    constexpr int NSUB = NT / 16;
+    constexpr int PPT = L::PPT;
+    constexpr int PPG = WM * PPW;                       // panels of the plan's group
+    constexpr int RED = WM * NSUB * 1024;               // one panel of a K group's waves (bytes)
+    static_assert(Elt<AT>::ES == 2, "block units take 16-bit A");
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int h = wv / WM, w = wv - h * WM;
+    const int tid = w * 64 + lane;
+    uint8_t *smem = smem_all + h * 2 * L::BTILE;
+    const int nch = u_hi - u_lo;
+    const int c_lo = u_lo + (nch * h) / KS, c_hi = u_lo + (nch * (h + 1)) / KS;
+    const int iters = (nch + KS - 1) / KS;              // barrier steps of every K group
+    const int n0 = tile * NT;
+    const Rsrc r_b = make_rsrc(p.B, p.b_bytes);
+    const Rsrc r_cols = make_rsrc(p.chunk_cols, (uint32_t)p.n_chunks * 128u);
+    const Rsrc r_a = make_rsrc(p.a, (uint32_t)p.n_chunks * (uint32_t)(PPG * 1024));
+    const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
+    // this thread's B pieces: row pk, 8-column segment, and where they go in LDS
+    int pk[PPT];
+    uint32_t pcol[PPT], plds[PPT];
+    bool pok[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const int pc = tid + j * WM * 64;
+        pk[j] = pc / L::SEGS;
+        const int seg = pc - pk[j] * L::SEGS;
+        const int ncol = n0 + 8 * seg;
+        pok[j] = ncol < p.N;
+        pcol[j] = (uint32_t)ncol * RB::BES;
+        plds[j] = (uint32_t)(pk[j] * L::LDB + 8 * seg) * 2u;
+    }
+    const int boff = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * L::LDB + 4 * (lane & 3)) * 2;
+    const uint32_t a_lane = (uint32_t)(w * PPW) * 1024u + (uint32_t)lane * 16u;
+
+    f32x4 acc[PPW][NSUB];
+#pragma unroll
+    for (int i = 0; i < PPW; i++)
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) acc[i][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    struct Stage {
+        typename RB::Raw rows[PPT];
+        Oct aop[PPW];
+    };
+    int ix[D][PPT];
+    Stage st[D];
+    auto load_index = [&](int c, int (&x)[PPT]) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) x[j] = (int)buf_b32(r_cols, ((uint32_t)c * 32u + (uint32_t)pk[j]) * 4u);
+    };
+    auto load_data = [&](int c, const int (&x)[PPT], Stage &sg) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            const bool ok = x[j] >= 0 && pok[j];
+            sg.rows[j] = RB::load(r_b, ok ? (uint32_t)x[j] * ldb_bytes + pcol[j] : kOOB);
+        }
+#pragma unroll
+        for (int i = 0; i < PPW; i++)
+            sg.aop[i] = buf_b128(r_a, (uint32_t)c * (uint32_t)(PPG * 1024) + a_lane + (uint32_t)i * 1024u);
+    };
+    // Every K group runs `iters` barrier steps (the shorter ones idle in their
+    // last); loads past a range re-read its last chunk (or, for an empty range,
+    // its first, always a chunk of the plan or out of range for the buffer
+    // descriptors, which read zeros), so every load is unconditional.
+    const int c_last = max(c_hi - 1, c_lo);
+#pragma unroll
+    for (int s = 0; s < D; s++) load_index(min(c_lo + s, c_last), ix[s]);
+#pragma unroll
+    for (int s = 0; s < D; s++) {
+        load_data(min(c_lo + s, c_last), ix[s], st[s]);
+        load_index(min(c_lo + s + D, c_last), ix[s]);
+    }
+    vm_drain();
+    for (int k0 = 0; k0 < iters; k0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; s++) {
+            const int k = k0 + s;
+            const int c = c_lo + k;
+            if (k < iters) {
+                uint8_t *buf = smem + (k & 1) * L::BTILE;
+#pragma unroll
+                for (int j = 0; j < PPT; j++) *reinterpret_cast<Oct *>(buf + plds[j]) = RL::cvt(st[s].rows[j]);
+                __syncthreads();
+                if (c < c_hi) {
+#pragma unroll
+                    for (int sb = 0; sb < NSUB; sb++) {
+                        const uint2 lo = lds_read_tr16(buf, boff + 32 * sb);
+                        const uint2 hi = lds_read_tr16(buf, boff + 32 * sb + 4 * L::LDB * 2);
+                        const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
+#pragma unroll
+                        for (int i = 0; i < PPW; i++) acc[i][sb] = Elt<AT>::mma(st[s].aop[i], b, acc[i][sb]);
+                    }
+                }
+            }
+            load_data(min(c + D, c_last), ix[s], st[s]);
+            load_index(min(c + 2 * D, c_last), ix[s]);
+        }
+    }
+    if constexpr (KS > 1) {
+        // K groups 1..KS-1 hand their partial tiles to group 0, one panel at a
+        // time (the sum runs in K-group order: deterministic)
+#pragma unroll
+        for (int i = 0; i < PPW; i++) {
+            __syncthreads();
+            if (h > 0) {
+#pragma unroll
+                for (int sb = 0; sb < NSUB; sb++)
+                    reinterpret_cast<f32x4 *>(smem_all + (h - 1) * RED + (w * NSUB + sb) * 1024)[lane] = acc[i][sb];
+            }
+            __syncthreads();
+            if (h == 0) {
+#pragma unroll
+                for (int q = 1; q < KS; q++)
+#pragma unroll
+                    for (int sb = 0; sb < NSUB; sb++)
+                        acc[i][sb] += reinterpret_cast<const f32x4 *>(smem_all + (q - 1) * RED + (w * NSUB + sb) * 1024)[lane];
+            }
+        }
+    }
+    if (h == 0) {
+        // D[4*(lane/16)+j][lane%16] of each 16x16 tile -> C
+        const int rbase = g * (PPG * 16) + w * PPW * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < PPW; i++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = rbase + 16 * i + j;
+                if (row >= p.n_rows) continue;
+                const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
+                float *crow = p.C + (long)row * p.ldc;
+#pragma unroll
+                for (int s = 0; s < NSUB; s++) {
+                    const int n = n0 + 16 * s + (lane & 15);
+                    if (n < p.N) {
+                        float v = acc[i][s][j] * rs;
+                        if (p.col_scale) v *= gbl(p.col_scale)[n];
+                        if (atomic) atomic_add_f32(crow + n, v);
+                        else gbl(crow)[n] = v;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();   // LDS is clean for the workgroup's next unit
+}
+
+// Standalone launch of the block unit: workgroup -> (unit, N-tile) by xcd_work.
+template <int PPW, int NT, int WM, int KS, tca_dtype AT, tca_dtype BT>
+__global__ __launch_bounds__(64 * WM * KS) void spmm_block_kernel(SpmmParams p)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t smem[block_lds_bytes<PPW, NT, WM, KS>()];
+    int ui, tile;
+    if (!xcd_work(p, blockIdx.x, ui, tile)) return;
+    const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
+    spmm_block_unit<PPW, NT, WM, KS, 2, AT, BT>(p, u.x, tile, u.z, u.w, p.split != 0, smem);
+}
+
 using SpmmFn = void (*)(SpmmParams);
 
 template <int RPW, tca_dtype AT, tca_dtype BT, bool IND, bool DN = false>
@@ -836,14 +1032,29 @@ SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
     return nullptr;
 }
 
-// Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A).
+// Block kernels on panel-dense tiles of 128 / 256-row plans: 4 waves per K
+// group (2 / 4 panels each), two K groups per workgroup.
+constexpr int kBlockWaves = 4, kBlockKS = 2;
+
+template <tca_dtype AT, tca_dtype BT>
+SpmmFn pick_block(int rpw, int nt)
+{
+    if (rpw == 128 && nt == 64) return spmm_block_kernel<2, 64, kBlockWaves, kBlockKS, AT, BT>;
+    if (rpw == 128 && nt == 128) return spmm_block_kernel<2, 128, kBlockWaves, kBlockKS, AT, BT>;
+    if (rpw == 256 && nt == 64) return spmm_block_kernel<4, 64, kBlockWaves, kBlockKS, AT, BT>;
+    if (rpw == 256 && nt == 128) return spmm_block_kernel<4, 128, kBlockWaves, kBlockKS, AT, BT>;
+    return nullptr;
+}
+
+// Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A on
+// the block kernels).
 SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt)
 {
     if (rpw >= 128) {
-        if (at == TCA_F16 && bt == TCA_F16) return pick_wide_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
-        if (at == TCA_BF16 && bt == TCA_BF16) return pick_wide_rpw<TCA_BF16, TCA_BF16, false, true>(rpw, nt);
-        if (at == TCA_F16 && bt == TCA_F32) return pick_wide_rpw<TCA_F16, TCA_F32, false, true>(rpw, nt);
-        if (at == TCA_BF16 && bt == TCA_F32) return pick_wide_rpw<TCA_BF16, TCA_F32, false, true>(rpw, nt);
+        if (at == TCA_F16 && bt == TCA_F16) return pick_block<TCA_F16, TCA_F16>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_BF16) return pick_block<TCA_BF16, TCA_BF16>(rpw, nt);
+        if (at == TCA_F16 && bt == TCA_F32) return pick_block<TCA_F16, TCA_F32>(rpw, nt);
+        if (at == TCA_BF16 && bt == TCA_F32) return pick_block<TCA_BF16, TCA_F32>(rpw, nt);
         return nullptr;
     }
     if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
@@ -1242,8 +1453,10 @@ __global__ void warm_up_kernel() {}
 // panel-dense f16 A (forward or transposed) with an f16 B on 16-row plans with
 // tile_n 16/32/64 (12..14), 32-row plans with tile_n 32/64 (18, 19) and 64-row
 // plans with tile_n 32 (20), with an f32 B on 16-row plans with tile_n 16/32/64
-// (15..17) and 32-row plans with tile_n 32 (21); SDDMM with an f32 (5) or f16
-// (6) G on the plan's 64-row SDDMM tiling.
+// (15..17) and 32-row plans with tile_n 32 (21); block units (the workgroup's
+// four waves share each B chunk tile) on 64-row (22) and 128-row (23) plans
+// with tile_n 64 and an f16 B; SDDMM with an f32 (5) or f16 (6) G on the plan's
+// 64-row SDDMM tiling.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1305,17 +1518,16 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
 // The batch kernel's workgroup: kBatchWaves waves share every unit (SpMM:
 // chunk range split + LDS reduction; SDDMM: contraction split over N).
 constexpr int kBatchWaves = 4;
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int kBatchSlot = cmax(cmax(SpmmLds<16, 64, TCA_F16>::SLOT, SpmmLds<32, 32, TCA_F16>::SLOT),   // largest SpMM
                                 cmax(SpmmLds<32, 64, TCA_F16>::SLOT, SpmmLds<64, 32, TCA_F16>::SLOT));  // variant's per-wave LDS
-constexpr int kBatchLds = kBatchWaves * kBatchSlot > ShLds<kBatchSddmmCB>::BYTES ? kBatchWaves * kBatchSlot
-                                                                                 : ShLds<kBatchSddmmCB>::BYTES;
+constexpr int kBatchLds = cmax(cmax(kBatchWaves * kBatchSlot, ShLds<kBatchSddmmCB>::BYTES),
+                                cmax(block_lds_bytes<1, 64, kBatchWaves, 1>(), block_lds_bytes<2, 64, kBatchWaves, 1>()));
 
 // DENSE_ONLY: the SpMM cases of panel-dense items only (and the SDDMM); the
 // register budget is then that of the lean dense loop, which keeps four
 // workgroups (16 waves) on each CU instead of three.
 template <bool DENSE_ONLY>
-__global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItemDev *__restrict__ items,
+__global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(DENSE_ONLY ? 4 : 3))) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                                  const BatchUnitDev *__restrict__ units, int n_units,
                                                                  int *counters, const int *__restrict__ part_off,
                                                                  int nparts)
@@ -1367,6 +1579,8 @@ __global__ __launch_bounds__(64 * kBatchWaves) void batch_kernel(const BatchItem
         case 19: spmm_wg_unit<32, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 20: spmm_wg_unit<64, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
         case 21: spmm_wg_unit<32, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 22: spmm_block_unit<1, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
+        case 23: spmm_block_unit<2, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
         case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
         case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
@@ -1411,6 +1625,9 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
         if (rpw == 32 && tile_n == 32) return f32 ? 21 : 18;
         if (!f32 && rpw == 32 && tile_n == 64) return 19;
         if (!f32 && rpw == 64 && tile_n == 32) return 20;
+        // block units: the workgroup's four waves share each B chunk tile
+        if (!f32 && rpw == 64 && tile_n == 64) return 22;
+        if (!f32 && rpw == 128 && tile_n == 64) return 23;
         return -1;
     }
     if (op == TCA_OP_SPMM) {
@@ -1579,7 +1796,12 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     const int maxc = t.max_group_chunks;
     int W = 1, S = 1;
     size_t lds_bytes = 0;
-    if (t.rpw >= 128) {
+    if (t.rpw >= 128 && dense) {
+        // block kernel: two K groups of four waves share each unit (no split:
+        // units that add atomically into C cost more than the parallelism gains)
+        W = kBlockWaves * kBlockKS;
+        lds_bytes = 0;
+    } else if (t.rpw >= 128) {
         // wide groups: W = rows / 32 waves share each B tile; split long groups
         // until ~kWideTargetGroups workgroups exist (pieces of >= 8 chunks)
         W = t.rpw / 32;
@@ -1616,7 +1838,7 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     const double a_bytes = dense ? (double)dense_bytes(t, a_dtype) : (double)t.nnz * (es_a + 2 + (p->transposed ? 4 : 0));
     const double b_bytes = (double)t.n_cols * N * (b_dtype == TCA_F32 ? 4 : b_dtype == TCA_E4M3 ? 1 : 2);
     const int tiles_per_xcd_pass = std::min(prm.n_tiles, 8);
-    if (t.rpw < 128 && prm.n_tiles > 1 && a_bytes * (tiles_per_xcd_pass - 1) > b_bytes * 7) {
+    if ((t.rpw < 128 || dense) && prm.n_tiles > 1 && a_bytes * (tiles_per_xcd_pass - 1) > b_bytes * 7) {
         prm.xcd_share = -2;
         grid = 8L * ((n_units + 7) / 8) * prm.n_tiles;
     } else if (prm.n_tiles % 8 == 0) {
