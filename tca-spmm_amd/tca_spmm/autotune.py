@@ -10,6 +10,12 @@ to this file; `choice_for()` looks up the nearest recorded shape (same M, K,
 N class, closest density) and falls back to a density rule when none exists.
 Entries are kept per A layout: tiled values, or panel-dense tiles
 (`layout="dense"`).
+
+`tune_batch()` chooses, layer by layer, the batched variants of the fused
+training step (forward and dX of every layer inside one persistent launch) by
+coordinate descent on the measured time of the whole batch; `record_batch()`
+stores them as entries marked "batch", which `choice_for(batch=True,
+dense=True)` returns for that layer and op.
 """
 import json
 import math
@@ -44,8 +50,23 @@ def heuristic(M, K, nnz, N):
     return rpw, tile
 
 
-# (rows_per_wave -> tile_n values) compiled into the persistent batch kernel
-BATCH_VARIANTS = {"spmm": {16: (16, 32, 64), 32: (32,)}, "spmm_t": {16: (16, 32, 64)}}
+# (rows_per_wave, tile_n) of the persistent batch kernel's panel-dense SpMM
+# variants with an F16 B (an F32 B is read through the batch's F16 copy): one
+# wave per 16 / 32 / 64 rows, or block units (the workgroup's four waves share
+# each B chunk tile) on 64- and 128-row plans with 64 columns
+BATCH_DENSE_CANDIDATES = ((16, 16), (16, 32), (16, 64), (32, 32), (32, 64), (64, 32), (64, 64), (128, 64))
+
+
+def _nearest(entries, op, M, K, N, d, layout, batch):
+    best, best_dist = None, None
+    for e in entries:
+        if (e.get("op", "spmm") != op or e["M"] != M or e["K"] != K or e.get("layout", "tiled") != layout
+                or bool(e.get("batch", False)) != batch):
+            continue
+        dist = abs(math.log(max(e["density"], 1e-6)) - math.log(max(d, 1e-6))) + abs(math.log(e["N"] / N))
+        if best is None or dist < best_dist:
+            best, best_dist = e, dist
+    return best if best is not None and best_dist < 0.7 else None
 
 
 def choice_for(M, K, nnz, N, op="spmm", batch=False, dense=False):
@@ -54,32 +75,27 @@ def choice_for(M, K, nnz, N, op="spmm", batch=False, dense=False):
     (op "spmm_t"), with nnz nonzeros and N dense columns.  dense=True: for A
     as panel-dense tiles.
 
-    batch=True: a variant the persistent batch kernel has.  There the
-    parallelism comes from the many items, and every work unit pays a fixed
-    claim + pipeline-start cost, so few, long units win: tiled values on
-    16 rows x 64 columns; panel-dense tiles on 32 rows (each gathered B row
-    serves two MFMA panels) x 64 columns, forward and transposed (a batch
-    reads an F32 gradient through its own F16 copy, which has the 64-column
-    variant).  Measured faster than the per-call tuned choices on the
-    ten-product suite and the twelve-layer forward + backward step (there the
-    transposed products at 32 x 64 took the step from 193 to 164 us; 32 x 32
-    had been the best while the F32 gradient was gathered directly)."""
+    batch=True: a variant the persistent batch kernel has.  With dense=True
+    the layer's own entry recorded by tune_batch() (nearest density, same M,
+    K, N), else the rule: there the parallelism comes from the many items,
+    and every work unit pays a fixed claim + pipeline-start cost, so few,
+    long units win: tiled values on 16 rows x 64 columns; panel-dense tiles
+    on 32 rows (each gathered B row serves two MFMA panels) x 64 columns,
+    forward and transposed."""
+    d = nnz / float(max(1, M * K))
+    layout = "dense" if dense else "tiled"
     if batch:
+        if dense:
+            e = _nearest(_load()["entries"], op, M, K, N, d, layout, True)
+            if e is not None:
+                return e["rows_per_wave"], e["tile_n"]
         t64 = 64 if N >= 64 else (32 if N >= 32 else 16)
         if dense and N >= 32:
             return 32, t64
         return 16, t64
-    d = nnz / float(max(1, M * K))
-    layout = "dense" if dense else "tiled"
-    best, best_dist = None, None
-    for e in _load()["entries"]:
-        if e.get("op", "spmm") != op or e["M"] != M or e["K"] != K or e.get("layout", "tiled") != layout:
-            continue
-        dist = abs(math.log(max(e["density"], 1e-6)) - math.log(max(d, 1e-6))) + abs(math.log(e["N"] / N))
-        if best is None or dist < best_dist:
-            best, best_dist = e, dist
-    if best is not None and best_dist < 0.7:
-        return best["rows_per_wave"], best["tile_n"]
+    e = _nearest(_load()["entries"], op, M, K, N, d, layout, False)
+    if e is not None:
+        return e["rows_per_wave"], e["tile_n"]
     return heuristic(M, K, nnz, N)
 
 
@@ -137,11 +153,116 @@ def record(A, N, results, op="spmm", path=None, layout="tiled"):
     best = results[0]
     M, K = (A.n_cols, A.n_rows) if op == "spmm_t" else (A.n_rows, A.n_cols)
     tab["entries"] = [e for e in tab["entries"]
-                      if not (e.get("op", "spmm") == op and e["M"] == M and e["K"] == K and e["N"] == N
-                              and abs(e["density"] - d) < 1e-3 and e.get("layout", "tiled") == layout)]
+                      if e.get("batch") or not (e.get("op", "spmm") == op and e["M"] == M and e["K"] == K and e["N"] == N
+                                                and abs(e["density"] - d) < 1e-3 and e.get("layout", "tiled") == layout)]
     tab["entries"].append({"op": op, "layout": layout, "M": M, "K": K, "N": N, "density": round(d, 5),
                            "rows_per_wave": best["rows_per_wave"], "tile_n": best["tile_n"], "us": best["us"],
                            "candidates": results})
+    with open(path or TABLE_PATH, "w") as f:
+        json.dump(tab, f, indent=1)
+
+
+def tune_batch(cases, candidates=BATCH_DENSE_CANDIDATES, iters=20, passes=1, device="cuda", log=None):
+    """Per-layer choice of the fused training step's batched SpMM variants:
+    forward (op "spmm") and dX (op "spmm_t") of every layer, by coordinate
+    descent on the measured time of the whole batch run (prologue packing the
+    dense tiles from the weight values + the persistent launch over forward,
+    dX and dW of all layers).  cases: workloads.SpmmCase with extra["G"].
+    Returns (choices, us): choices[i] = {"fwd": (rpw, tile), "dx": (rpw, tile)}
+    and the step's batch time with them; every candidate tried is timed."""
+    import numpy as np
+    import torch
+
+    from .batch import Batch, sddmm_item, spmm_item
+    from .core import Plan
+
+    layers = []
+    for c in cases:
+        W = c.A
+        v = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(device)
+        p16 = Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+        layers.append(dict(case=c, v=v, plans={16: p16}, tplans={},
+                           X=torch.from_numpy(c.B.view(np.int16)).view(torch.float16).to(device),
+                           G=torch.from_numpy(c.extra["G"]).to(device),
+                           Y=torch.empty((W.n_rows, c.N), dtype=torch.float32, device=device),
+                           dX=torch.empty((W.n_cols, c.N), dtype=torch.float32, device=device)))
+
+    def plan(L, r):
+        if r not in L["plans"]:
+            W = L["case"].A
+            L["plans"][r] = Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, r)
+        return L["plans"][r]
+
+    def tplan(L, r):
+        if r not in L["tplans"]:
+            L["tplans"][r] = L["plans"][16].transpose(r)
+        return L["tplans"][r]
+
+    def measure(choice):
+        items, keep = [], []
+        for L, ch in zip(layers, choice):
+            pw = plan(L, ch["fwd"][0])
+            pt = tplan(L, ch["dx"][0])
+            w = pw.pack(L["v"])
+            dW = torch.empty(max(L["case"].A.nnz, 1), dtype=torch.float32, device=device)
+            keep += [w, dW]
+            items += [spmm_item(pw, w, L["X"], L["Y"], ch["fwd"][1], dense="pack", values_order=pw),
+                      spmm_item(pt, w, L["G"], L["dX"], ch["dx"][1], dense="pack", values_order=pw),
+                      sddmm_item(pw, L["G"], L["X"], dW)]
+        b = Batch(items)
+        for _ in range(3):
+            b.run()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if hasattr(torch.cuda, "_sleep"):
+            torch.cuda._sleep(2_000_000)
+        s.record()
+        for _ in range(iters):
+            b.run()
+        e.record()
+        torch.cuda.synchronize()
+        b.close()
+        return s.elapsed_time(e) * 1e3 / iters
+
+    choice = []
+    for L in layers:
+        W, N = L["case"].A, L["case"].N
+        t = 64 if N >= 64 else 32
+        choice.append({"fwd": (32, t), "dx": (32, t)})
+    best = measure(choice)
+    if log:
+        log(f"batch start {best:.1f} us")
+    for _ in range(passes):
+        for i in range(len(layers)):
+            for op in ("fwd", "dx"):
+                for cand in candidates:
+                    if tuple(cand) == tuple(choice[i][op]) or cand[1] > max(16, layers[i]["case"].N):
+                        continue
+                    trial = [dict(c) for c in choice]
+                    trial[i][op] = tuple(cand)
+                    us = measure(trial)
+                    if us < best * 0.995:
+                        best, choice = us, trial
+                if log:
+                    log(f"layer {i} {op}: {choice[i][op]} -> {best:.1f} us")
+    return choice, best
+
+
+def record_batch(cases, choices, us, path=None):
+    """Store the batch choices (one entry per layer and op, marked batch) in the
+    tuning table; choice_for(batch=True, dense=True) reads them back."""
+    tab = _load()
+    for c, ch in zip(cases, choices):
+        A, N = c.A, c.N
+        d = A.nnz / float(A.n_rows * A.n_cols)
+        for op, key in (("spmm", "fwd"), ("spmm_t", "dx")):
+            M, K = (A.n_cols, A.n_rows) if op == "spmm_t" else (A.n_rows, A.n_cols)
+            tab["entries"] = [e for e in tab["entries"]
+                              if not (e.get("batch") and e.get("op", "spmm") == op and e["M"] == M and e["K"] == K
+                                      and e["N"] == N and abs(e["density"] - d) < 1e-3)]
+            tab["entries"].append({"op": op, "layout": "dense", "batch": True, "M": M, "K": K, "N": N,
+                                   "density": round(d, 5), "rows_per_wave": int(ch[key][0]), "tile_n": int(ch[key][1]),
+                                   "us": us})
     with open(path or TABLE_PATH, "w") as f:
         json.dump(tab, f, indent=1)
 
@@ -163,10 +284,19 @@ def main(argv=None):
     ap.add_argument("--workloads", default="plain_gpu,batch12,precision,sparse_linear")
     ap.add_argument("--iters", type=int, default=30)
     ap.add_argument("--layouts", default="tiled,dense")
+    ap.add_argument("--batch", action="store_true", help="also tune the batched training step of batch12")
+    ap.add_argument("--batch-only", action="store_true", help="only tune the batched training step (keeps the table)")
     args = ap.parse_args(argv)
     global _table
-    _table = {"entries": []}
     dev = torch.device("cuda:0")
+    if args.batch_only:
+        _table = None
+        _load()
+        cases = workloads.batch12()
+        ch, us = tune_batch(cases, log=lambda m: print(m, flush=True))
+        record_batch(cases, ch, us, path=args.out)
+        return
+    _table = {"entries": []}
 
     def f16(bits):
         return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
@@ -195,6 +325,10 @@ def main(argv=None):
                 record(A, G.shape[1], rt, op="spmm_t", path=args.out, layout=layout)
                 print(f"{layout:5s} spmm_t {A.name:40s} N={G.shape[1]:5d} best rpw={rt[0]['rows_per_wave']} "
                       f"tile={rt[0]['tile_n']} {rt[0]['us']:.1f} us (worst {rt[-1]['us']:.1f})", flush=True)
+    if args.batch:
+        cases = workloads.batch12()
+        ch, us = tune_batch(cases, log=lambda m: print(m, flush=True))
+        record_batch(cases, ch, us, path=args.out)
 
 
 if __name__ == "__main__":
