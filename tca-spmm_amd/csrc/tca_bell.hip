@@ -55,7 +55,7 @@ __global__ __launch_bounds__(64) void bell_spmm_kernel(BellParams p)
         Oct aop[PPG];
         int blk;                       // block column of this lane's k rows (-1: none)
     };
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[32 * LDB * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[32 * LDB * 2];   // rows by tr_row
 
     const int unit = blockIdx.x;
     if (unit >= p.n_units) return;
@@ -114,13 +114,13 @@ __global__ __launch_bounds__(64) void bell_spmm_kernel(BellParams p)
 #pragma unroll
         for (int j = 0; j < KPL; j++) rows[j] = RL::cvt(st.rows[j]);
 #pragma unroll
-        for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = rows[j];
+        for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + tr_row(k0 + j, LDB * 2) + 16 * seg) = rows[j];
         wave_lds_sync();
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
-            const int off = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
-            const uint2 lo = lds_read_tr16(lds_b, off);
-            const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+            const int row = 8 * (lane >> 4) + ((lane & 15) >> 2), col = (16 * s + 4 * (lane & 3)) * 2;
+            const uint2 lo = lds_read_tr16(lds_b, tr_row(row, LDB * 2) + col);
+            const uint2 hi = lds_read_tr16(lds_b, tr_row(row + 4, LDB * 2) + col);
             const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
 #pragma unroll
             for (int i = 0; i < PPG; i++) acc[i][s] = Elt<AT>::mma(st.aop[i], b, acc[i][s]);

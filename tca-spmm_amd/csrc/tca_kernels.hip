@@ -261,7 +261,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int j = 0; j < KPL; j++) rows[j] = RL::cvt(st.rows[j]);
         if constexpr (TR) {
 #pragma unroll
-            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = rows[j];
+            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + tr_row(k0 + j, LDB * 2) + 16 * seg) = rows[j];
         } else {
 #pragma unroll
             for (int e = 0; e < 8; e++) {
@@ -275,9 +275,9 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int s = 0; s < NSUB; s++) {
             Oct b;
             if constexpr (TR) {
-                const int off = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
-                const uint2 lo = lds_read_tr16(lds_b, off);
-                const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+                const int row = 8 * (lane >> 4) + ((lane & 15) >> 2), col = (16 * s + 4 * (lane & 3)) * 2;
+                const uint2 lo = lds_read_tr16(lds_b, tr_row(row, LDB * 2) + col);
+                const uint2 hi = lds_read_tr16(lds_b, tr_row(row + 4, LDB * 2) + col);
                 b = make_uint4(lo.x, lo.y, hi.x, hi.y);
             } else {
                 b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
@@ -302,7 +302,7 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int j = 0; j < KPL; j++) rows[j] = RL::cvt(st.rows[j]);
         if constexpr (TR) {
 #pragma unroll
-            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = rows[j];
+            for (int j = 0; j < KPL; j++) *reinterpret_cast<Oct *>(lds_b + tr_row(k0 + j, LDB * 2) + 16 * seg) = rows[j];
         } else {
 #pragma unroll
             for (int e = 0; e < 8; e++) {
@@ -332,9 +332,9 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
         for (int s = 0; s < NSUB; s++) {
             Oct b;
             if constexpr (TR) {
-                const int off = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
-                const uint2 lo = lds_read_tr16(lds_b, off);
-                const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+                const int row = 8 * (lane >> 4) + ((lane & 15) >> 2), col = (16 * s + 4 * (lane & 3)) * 2;
+                const uint2 lo = lds_read_tr16(lds_b, tr_row(row, LDB * 2) + col);
+                const uint2 hi = lds_read_tr16(lds_b, tr_row(row + 4, LDB * 2) + col);
                 b = make_uint4(lo.x, lo.y, hi.x, hi.y);
             } else {
                 b = reinterpret_cast<const Oct *>(lds_b)[b_oct(16 * s + (lane & 15), lane >> 4)];
@@ -528,7 +528,7 @@ constexpr int kWideEP = 2;   // prefetched entries per lane per panel (128 of a 
 template <int NT, int WM, int PPW>
 struct WideLds {
     static constexpr int LDB = NT + 16;                          // B row pitch (elements)
-    static constexpr int BTILE = 32 * LDB * 2;                   // bytes per chunk tile
+    static constexpr int BTILE = 32 * LDB * 2;                   // bytes per chunk tile (rows by tr_row)
     static constexpr int ATILE = PPW * 512 * 2;                  // bytes of one wave's A tiles
     static constexpr int BYTES = 2 * BTILE + WM * ATILE;
     static constexpr int SEGS = NT / 8;                          // 16-byte pieces per B row
@@ -658,15 +658,15 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         for (int j = 0; j < L::PPT; j++)
             if (pvalid[j]) {
                 const Oct v = RL::cvt(st.rows[j]);
-                *reinterpret_cast<Oct *>(buf + (pk[j] * L::LDB + 8 * pseg[j]) * 2) = v;
+                *reinterpret_cast<Oct *>(buf + tr_row(pk[j], L::LDB * 2) + 16 * pseg[j]) = v;
             }
     };
     auto compute_dense = [&](const Stage &st, const uint8_t *buf) {
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
-            const int off = (brow * L::LDB + 16 * s + bcol) * 2;
-            const uint2 lo = lds_read_tr16(buf, off);
-            const uint2 hi = lds_read_tr16(buf, off + 4 * L::LDB * 2);
+            const int col = (16 * s + bcol) * 2;
+            const uint2 lo = lds_read_tr16(buf, tr_row(brow, L::LDB * 2) + col);
+            const uint2 hi = lds_read_tr16(buf, tr_row(brow + 4, L::LDB * 2) + col);
             const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
 #pragma unroll
             for (int i = 0; i < PPW; i++) acc[i][s] = Elt<AT>::mma(st.aop[i], b, acc[i][s]);
@@ -701,9 +701,9 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         for (int i = 0; i < PPW; i++) a[i] = reinterpret_cast<const Oct *>(lds_a)[i * 64 + a_oct(lane & 15, lane >> 4)];
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
-            const int off = (brow * L::LDB + 16 * s + bcol) * 2;
-            const uint2 lo = lds_read_tr16(buf, off);
-            const uint2 hi = lds_read_tr16(buf, off + 4 * L::LDB * 2);
+            const int col = (16 * s + bcol) * 2;
+            const uint2 lo = lds_read_tr16(buf, tr_row(brow, L::LDB * 2) + col);
+            const uint2 hi = lds_read_tr16(buf, tr_row(brow + 4, L::LDB * 2) + col);
             const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
 #pragma unroll
             for (int i = 0; i < PPW; i++) acc[i][s] = Elt<AT>::mma(a[i], b, acc[i][s]);
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 template <int NT, int WM>
 struct BlockLds {
     static constexpr int LDB = NT + 16;                  // B row pitch (elements)
-    static constexpr int BTILE = 32 * LDB * 2;           // one chunk's B tile (bytes)
+    static constexpr int BTILE = 32 * LDB * 2;           // one chunk's B tile (bytes, rows by tr_row)
     static constexpr int SEGS = NT / 8;                  // 16-byte pieces per B row
     static constexpr int PPT = 32 * SEGS / (WM * 64);    // pieces per thread
     static_assert(PPT >= 1 && (32 * SEGS) % (WM * 64) == 0, "B pieces must divide over the waves");
@@ -841,9 +841,10 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
         const int ncol = n0 + 8 * seg;
         pok[j] = ncol < p.N;
         pcol[j] = (uint32_t)ncol * RB::BES;
-        plds[j] = (uint32_t)(pk[j] * L::LDB + 8 * seg) * 2u;
+        plds[j] = (uint32_t)(tr_row(pk[j], L::LDB * 2) + 16 * seg);
     }
-    const int boff = ((8 * (lane >> 4) + ((lane & 15) >> 2)) * L::LDB + 4 * (lane & 3)) * 2;
+    const int brow = 8 * (lane >> 4) + ((lane & 15) >> 2);
+    const int boff_lo = tr_row(brow, L::LDB * 2) + 8 * (lane & 3), boff_hi = tr_row(brow + 4, L::LDB * 2) + 8 * (lane & 3);
     const uint32_t a_lane = (uint32_t)(w * PPW) * 1024u + (uint32_t)lane * 16u;
 
     f32x4 acc[PPW][NSUB];
@@ -898,8 +899,8 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
                 if (c < c_hi) {
 #pragma unroll
                     for (int sb = 0; sb < NSUB; sb++) {
-                        const uint2 lo = lds_read_tr16(buf, boff + 32 * sb);
-                        const uint2 hi = lds_read_tr16(buf, boff + 32 * sb + 4 * L::LDB * 2);
+                        const uint2 lo = lds_read_tr16(buf, boff_lo + 32 * sb);
+                        const uint2 hi = lds_read_tr16(buf, boff_hi + 32 * sb);
                         const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
 #pragma unroll
                         for (int i = 0; i < PPW; i++) acc[i][sb] = Elt<AT>::mma(st[s].aop[i], b, acc[i][sb]);
@@ -1805,7 +1806,7 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
         // wide groups: W = rows / 32 waves share each B tile; split long groups
         // until ~kWideTargetGroups workgroups exist (pieces of >= 8 chunks)
         W = t.rpw / 32;
-        lds_bytes = 2 * 32 * (tile_n + 16) * 2 + W * 2 * 512 * 2;
+        lds_bytes = 2 * 32 * (tile_n + 16) * 2 + W * 2 * 512 * 2;   // = WideLds<...>::BYTES
         if (base < kWideTargetGroups) S = (int)std::min<long>((kWideTargetGroups + base - 1) / base, std::max(1, maxc / 8));
     } else {
         const int es = a_dtype == TCA_E4M3 ? 1 : 2;

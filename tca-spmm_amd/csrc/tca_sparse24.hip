@@ -131,7 +131,7 @@ __global__ __launch_bounds__(64) void spmm24_kernel(S24Params p)
         Oct aop[PPG];
         uint32_t idx[PPG];
     };
-    __shared__ __attribute__((aligned(16))) uint8_t lds_b[64 * LDB * 2];
+    __shared__ __attribute__((aligned(16))) uint8_t lds_b[64 * LDB * 2];   // rows by tr_row
 
     const int unit = blockIdx.x;
     if (unit >= p.n_units) return;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(64) void spmm24_kernel(S24Params p)
     auto compute = [&](const Stage &st) {
 #pragma unroll
         for (int j = 0; j < KPL; j++)
-            *reinterpret_cast<Oct *>(lds_b + ((k0 + j) * LDB + 8 * seg) * 2) = RL::cvt(st.rows[j]);
+            *reinterpret_cast<Oct *>(lds_b + tr_row(k0 + j, LDB * 2) + 16 * seg) = RL::cvt(st.rows[j]);
         wave_lds_sync();
         int idx[PPG];
 #pragma unroll
@@ -187,9 +187,9 @@ __global__ __launch_bounds__(64) void spmm24_kernel(S24Params p)
             uint4 b[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int off = ((32 * h + 8 * (lane >> 4) + ((lane & 15) >> 2)) * LDB + 16 * s + 4 * (lane & 3)) * 2;
-                const uint2 lo = lds_read_tr16(lds_b, off);
-                const uint2 hi = lds_read_tr16(lds_b, off + 4 * LDB * 2);
+                const int row = 32 * h + 8 * (lane >> 4) + ((lane & 15) >> 2), col = (16 * s + 4 * (lane & 3)) * 2;
+                const uint2 lo = lds_read_tr16(lds_b, tr_row(row, LDB * 2) + col);
+                const uint2 hi = lds_read_tr16(lds_b, tr_row(row + 4, LDB * 2) + col);
                 b[h] = make_uint4(lo.x, lo.y, hi.x, hi.y);
             }
 #pragma unroll
