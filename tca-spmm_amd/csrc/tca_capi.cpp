@@ -97,6 +97,25 @@ uint64_t pattern_digest(int n_rows, int n_cols, int nnz, const int32_t *row_offs
 
 namespace {
 
+// Every table a launch on the plan reads: the SpMM unit tables and, for a
+// forward plan, its SDDMM tiling with the SDDMM unit tables.  Built here so
+// that tca_spmm / tca_sddmm / tca_batch_create never allocate or change a plan
+// (capturable from the first call, safe from several threads).
+tca_status prepare_launch_tables(tca_plan *p)
+{
+    tca_status s = tca::prepare_spmm_units(p);
+    if (s != TCA_OK || p->transposed) return s;
+    const tca_plan *aux = nullptr;
+    const int32_t *dst = nullptr;
+    s = tca::sddmm_tiling(p, &aux, &dst);
+    for (int cb = 4; s == TCA_OK && cb >= 1; cb >>= 1) {
+        const int32_t *units = nullptr;
+        int n = 0;
+        s = tca::sddmm_units(aux, cb, &units, &n);
+    }
+    return s;
+}
+
 tca_status plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offset, const int32_t *col_idx,
                        int rows_per_wave, tca_plan **plan, bool device)
 {
@@ -113,6 +132,7 @@ tca_status plan_create(int n_rows, int n_cols, int nnz, const int32_t *row_offse
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (device) {
         s = upload(p);
+        if (s == TCA_OK) s = prepare_launch_tables(p);
         if (s != TCA_OK) { tca_plan_destroy(p); return s; }
     }
     *plan = p;
@@ -171,6 +191,7 @@ extern "C" tca_status tca_plan_create_transpose(const tca_plan *fwd, int rows_pe
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (fwd->d_base) {  // a host-only forward plan gives a host-only transpose
         s = upload(p);
+        if (s == TCA_OK) s = prepare_launch_tables(p);
         if (s != TCA_OK) { tca_plan_destroy(p); return s; }
     }
     *plan_t = p;

@@ -562,8 +562,12 @@ tca_status build_tables(int n_rows, int n_cols, int nnz, const int32_t *row_offs
     if (n_rows <= 0 || n_cols <= 0 || nnz < 0 || !row_offset || (nnz > 0 && !col_idx))
         return fail(TCA_ERR_INVALID_ARG, "plan: bad CSR arguments");
     if (row_offset[0] != 0 || row_offset[n_rows] != nnz) return fail(TCA_ERR_FORMAT, "plan: row offsets do not span [0, nnz]");
+    // offsets first (monotone, inside [0, nnz]), so that the column pass below
+    // never reads past col_idx[nnz - 1]
+    for (int r = 0; r < n_rows; r++)
+        if (row_offset[r + 1] < row_offset[r] || row_offset[r + 1] > nnz)
+            return fail(TCA_ERR_FORMAT, "plan: row offsets decrease or pass nnz");
     for (int r = 0; r < n_rows; r++) {
-        if (row_offset[r + 1] < row_offset[r]) return fail(TCA_ERR_FORMAT, "plan: row offsets decrease");
         for (int e = row_offset[r]; e < row_offset[r + 1]; e++) {
             const int c = col_idx[e];
             if (c < 0 || c >= n_cols) return fail(TCA_ERR_FORMAT, "plan: column out of range");

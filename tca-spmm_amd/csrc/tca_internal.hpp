@@ -59,14 +59,15 @@ struct tca_plan {
     const int32_t *d_work_order = nullptr;
     // SDDMM: the 64-row tiling its kernel runs on (this plan itself when it has
     // 64-row groups) and, for each of that tiling's entries, its index in this
-    // plan's tiled order (null for this plan itself); built on first use
+    // plan's tiled order (null for this plan itself); built with the plan
     tca_plan *sddmm_aux = nullptr;
     int32_t *d_sddmm_dst = nullptr;
     // SDDMM work units (group, first chunk) of this plan as an SDDMM tiling,
-    // for 4-, 2- and 1-chunk blocks, built on first use
+    // for 4-, 2- and 1-chunk blocks, built with the plan
     void *d_sddmm_units[3] = {nullptr, nullptr, nullptr};
     int n_sddmm_units[3] = {0, 0, 0};
-    // SpMM unit tables (group, pieces, c_lo, c_hi), one per split factor
+    // SpMM unit tables (group, pieces, c_lo, c_hi), one per split factor, built
+    // with the plan (prepare_spmm_units)
     struct UnitTable {
         int pieces = 1, count = 0;
         bool split = false;
@@ -103,9 +104,13 @@ tca_status launch_quantize_rows(int n_rows, const int32_t *row_offset, const uin
 tca_status launch_quantize_cols(int K, int N, const uint16_t *B, int ldb, uint8_t *q, int ldq, float *scale,
                                 hipStream_t stream);
 int default_tile_n(const tca_plan *p, int N);
+// Unit tables of every split factor a standalone SpMM launch can pick, built
+// at plan creation (launches only look them up).
+tca_status prepare_spmm_units(tca_plan *p);
 tca_status launch_warm_up(hipStream_t stream);
 // The SDDMM tiling of a forward plan (tca_capi.cpp): *aux has 64-row groups,
-// *dst maps its entries to p's tiled order (null when *aux == p).
+// *dst maps its entries to p's tiled order (null when *aux == p).  Built, with
+// its unit tables, when the forward plan is created; later calls look it up.
 constexpr int kSddmmRows = 64;
 tca_status sddmm_tiling(const tca_plan *p, const tca_plan **aux, const int32_t **dst);
 // SDDMM work units (group, first chunk) of a 64-row tiling for blocks of cb (4,

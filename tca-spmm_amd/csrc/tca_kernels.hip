@@ -1709,7 +1709,6 @@ int default_tile_n(const tca_plan *p, int N)
 // chunks per wave are split into S units that accumulate atomically.
 static constexpr int kTargetWaves = 4096;
 static constexpr int kMaxWaveChunks = 32;
-static constexpr long kWideTargetGroups = 1024;
 
 static int spmm_slot_bytes(int rpw, int nt, int es)
 {
@@ -1719,39 +1718,58 @@ static int spmm_slot_bytes(int rpw, int nt, int es)
     return tiles > red ? tiles : red;
 }
 
-static tca_status spmm_units(const tca_plan *cp, int S, const int4 **units, int *n_units, bool *split)
+// Split factor of the launches' unit tables: a group of maxc chunks under W
+// waves per workgroup runs as S units (1 when the waves hold it whole).
+static int split_for(int maxc, int W) { return maxc > W * kMaxWaveChunks ? (maxc + W * kMaxWaveChunks - 1) / (W * kMaxWaveChunks) : 1; }
+
+// The unit table of split factor S: one (group, pieces, c_lo, c_hi) per unit,
+// heavy groups first.  Built for every S a launch can choose when the plan is
+// created (prepare_spmm_units), so that launches only look them up: tca_spmm
+// allocates nothing, can be captured in a graph from its first call and is
+// safe to call from several threads on one plan.
+static tca_status spmm_units(const tca_plan *p, int S, const int4 **units, int *n_units, bool *split)
 {
-    tca_plan *p = const_cast<tca_plan *>(cp);
-    const PlanTables &t = p->host;
-    for (auto &u : p->unit_tables)
+    for (const auto &u : p->unit_tables)
         if (u.pieces == S) {
             *units = static_cast<const int4 *>(u.dev);
             *n_units = u.count;
             *split = u.split;
             return TCA_OK;
         }
-    std::vector<int4> h;
-    bool any_split = false;
-    for (int gi = 0; gi < t.n_groups; gi++) {
-        const int g = t.work_order[gi];
-        const int cb = t.grp_chunk_ptr[g], nch = t.grp_chunk_ptr[g + 1] - cb;
-        const int P = std::max(1, std::min(S, nch / 4));
-        if (P > 1) any_split = true;
-        for (int k = 0; k < P; k++)
-            h.push_back(make_int4(g, P, cb + (int)((long)nch * k / P), cb + (int)((long)nch * (k + 1) / P)));
+    return fail(TCA_ERR_INVALID_ARG, "tca_spmm: the plan has no unit table for split factor " + std::to_string(S));
+}
+
+tca_status prepare_spmm_units(tca_plan *p)
+{
+    const PlanTables &t = p->host;
+    std::vector<int> splits = {1};
+    for (int W = 1; W <= 8; W *= 2) {
+        const int S = split_for(t.max_group_chunks, W);
+        if (std::find(splits.begin(), splits.end(), S) == splits.end()) splits.push_back(S);
     }
-    tca_plan::UnitTable ut;
-    ut.pieces = S;
-    ut.count = (int)h.size();
-    ut.split = any_split;
-    if (!h.empty()) {
-        TCA_HIP_TRY(hipMalloc(&ut.dev, sizeof(int4) * h.size()));
-        TCA_HIP_TRY(hipMemcpy(ut.dev, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    for (int S : splits) {
+        std::vector<int4> h;
+        bool any_split = false;
+        for (int gi = 0; gi < t.n_groups; gi++) {
+            const int g = t.work_order[gi];
+            const int cb = t.grp_chunk_ptr[g], nch = t.grp_chunk_ptr[g + 1] - cb;
+            const int P = std::max(1, std::min(S, nch / 4));
+            if (P > 1) any_split = true;
+            for (int k = 0; k < P; k++)
+                h.push_back(make_int4(g, P, cb + (int)((long)nch * k / P), cb + (int)((long)nch * (k + 1) / P)));
+        }
+        tca_plan::UnitTable ut;
+        ut.pieces = S;
+        ut.count = (int)h.size();
+        ut.split = any_split;
+        if (!h.empty()) {
+            if (hipMalloc(&ut.dev, sizeof(int4) * h.size()) != hipSuccess) return fail(TCA_ERR_ALLOC, "plan: unit table");
+            p->unit_tables.push_back(ut);   // owned by the plan from here (freed with it)
+            TCA_HIP_TRY(hipMemcpy(ut.dev, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+        } else {
+            p->unit_tables.push_back(ut);
+        }
     }
-    p->unit_tables.push_back(ut);
-    *units = static_cast<const int4 *>(ut.dev);
-    *n_units = ut.count;
-    *split = ut.split;
     return TCA_OK;
 }
 
@@ -1803,17 +1821,15 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
         W = kBlockWaves * kBlockKS;
         lds_bytes = 0;
     } else if (t.rpw >= 128) {
-        // wide groups: W = rows / 32 waves share each B tile; split long groups
-        // until ~kWideTargetGroups workgroups exist (pieces of >= 8 chunks)
+        // wide groups: W = rows / 32 waves share each B tile, one unit per group
         W = t.rpw / 32;
         lds_bytes = 2 * 32 * (tile_n + 16) * 2 + W * 2 * 512 * 2;   // = WideLds<...>::BYTES
-        if (base < kWideTargetGroups) S = (int)std::min<long>((kWideTargetGroups + base - 1) / base, std::max(1, maxc / 8));
     } else {
         const int es = a_dtype == TCA_E4M3 ? 1 : 2;
         const int slot = spmm_slot_bytes(t.rpw, tile_n, es);
         const int wcap = std::max(1, std::min(8, 65536 / slot));   // = SpmmLds<...>::WMAX
         while (W * 2 <= wcap && maxc / (W * 2) >= 4 && base * W < kTargetWaves) W *= 2;
-        S = maxc > W * kMaxWaveChunks ? (maxc + W * kMaxWaveChunks - 1) / (W * kMaxWaveChunks) : 1;
+        S = split_for(maxc, W);
         lds_bytes = (size_t)W * slot;
     }
 

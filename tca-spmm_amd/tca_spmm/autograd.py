@@ -5,7 +5,9 @@ Backward (hand-written, both on the C ABI):
   dW = SDDMM(G, X^T) sampled on W's pattern, FP32, in tiled order
 Both read G in the weight's MFMA type: an FP32 G is rounded to it once (round to
 nearest even, the same bits the kernels would produce rounding on load), which
-halves the bytes both kernels gather.
+halves the bytes both kernels gather.  X may be FP16 / BF16 (the weight's type)
+or FP32: the forward rounds an FP32 X on load and the SDDMM gets the same
+rounded X.
 W's values live in *tiled order* (the plan's entry order) as the parameter,
 so no permutation runs per step; `SparseLinear.csr_values()` exports them.
 dense=True runs the forward and dX on panel-dense tiles built from the
@@ -20,6 +22,8 @@ from .core import Plan, sddmm, spmm
 class SpmmFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, w_tiled, X, plan, plan_t, tile_n, dense=False):
+        if X.dim() == 2 and X.shape[1] > 1 and X.stride(1) != 1:
+            X = X.contiguous()
         if dense:
             Y = spmm(plan, plan.pack_dense(w_tiled), X, tile_n=tile_n, dense=True)
         else:
@@ -36,7 +40,10 @@ class SpmmFunction(torch.autograd.Function):
             G = G.to(w_tiled.dtype)
         dW = dX = None
         if ctx.needs_input_grad[0]:
-            dW = sddmm(ctx.plan, G, X).to(w_tiled.dtype)
+            # an FP32 X entered the forward rounded to the weight's type on load
+            # (nearest even); the SDDMM takes X in that type, so round it the same way
+            Xw = X if X.dtype == w_tiled.dtype else X.to(w_tiled.dtype)
+            dW = sddmm(ctx.plan, G, Xw).to(w_tiled.dtype)
         if ctx.needs_input_grad[1]:
             if ctx.dense:
                 dX = spmm(ctx.plan_t, ctx.plan_t.pack_dense(w_tiled), G, tile_n=ctx.tile_n, dense=True).to(X.dtype)

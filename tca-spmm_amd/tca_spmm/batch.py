@@ -21,7 +21,7 @@ import torch
 
 from . import _lib
 from ._lib import OP_SDDMM, OP_SPMM, call
-from .core import _stream, tca_dtype
+from .core import _F16ISH, _VALUE_DTYPES, _check_devices, _check_matrix, _check_vector, _stream, tca_dtype
 
 
 def spmm_item(plan, a_tiled, B, out, tile_n=0, dense=False, values_order=None):
@@ -41,8 +41,30 @@ def sddmm_item(plan, G, X, out):
     return dict(op=OP_SDDMM, plan=plan, a=G, b=X, out=out, tile_n=0)
 
 
+def _check_item(it):
+    """The checks tca_spmm / tca_sddmm make, per item (raises ValueError)."""
+    plan, a, b, out = it["plan"], it["a"], it["b"], it["out"]
+    if it["op"] == OP_SPMM:
+        _check_matrix(b, "B", rows=plan.n_cols, dtypes=_VALUE_DTYPES)
+        _check_matrix(out, "out", dtypes=(torch.float32,), min_rows=plan.n_rows, min_cols=b.shape[1])
+        d = it.get("dense", False)
+        if d is True:
+            if a.numel() != plan.dense_elems:
+                raise ValueError(f"dense A must hold plan.dense_elems = {plan.dense_elems} elements")
+            _check_vector(a, "dense A", plan.dense_elems, _VALUE_DTYPES)
+        elif plan.nnz:
+            _check_vector(a, "values", plan.nnz, _VALUE_DTYPES)
+    else:
+        _check_matrix(a, "G", rows=plan.n_rows, dtypes=_F16ISH + (torch.float32,))
+        _check_matrix(b, "X", rows=plan.n_cols, cols=a.shape[1], dtypes=_F16ISH)
+        _check_vector(out, "out", plan.nnz, (torch.float32,))
+    _check_devices(plan, a, b, out)
+
+
 class Batch:
     def __init__(self, items):
+        for it in items:
+            _check_item(it)
         arr = (_lib.BatchItem * len(items))()
         self._keep = []
         for i, it in enumerate(items):

@@ -40,6 +40,52 @@ def _stream(stream=None):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+# Argument checks done before any launch: the C ABI receives raw pointers and
+# leading dimensions only, so a column-strided view, an undersized output or a
+# tensor on another GPU would otherwise read or write the wrong memory.
+def _check_matrix(t, name, rows=None, cols=None, dtypes=None, min_rows=None, min_cols=None):
+    if not isinstance(t, torch.Tensor) or t.dim() != 2:
+        raise ValueError(f"{name} must be a 2-D tensor")
+    if t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError(f"{name} must have unit stride along its rows (stride(1) == 1), got {t.stride()}")
+    if t.stride(0) < t.shape[1]:
+        raise ValueError(f"{name} rows overlap (stride(0) {t.stride(0)} < {t.shape[1]} columns)")
+    if dtypes is not None and t.dtype not in dtypes:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected one of {sorted(str(d) for d in dtypes)}")
+    if rows is not None and t.shape[0] != rows:
+        raise ValueError(f"{name} has {t.shape[0]} rows, expected {rows}")
+    if cols is not None and t.shape[1] != cols:
+        raise ValueError(f"{name} has {t.shape[1]} columns, expected {cols}")
+    if min_rows is not None and t.shape[0] < min_rows:
+        raise ValueError(f"{name} has {t.shape[0]} rows, needs at least {min_rows}")
+    if min_cols is not None and t.shape[1] < min_cols:
+        raise ValueError(f"{name} has {t.shape[1]} columns, needs at least {min_cols}")
+
+
+def _check_vector(t, name, min_len, dtypes=None):
+    if not isinstance(t, torch.Tensor) or t.dim() != 1 or (t.numel() > 1 and t.stride(0) != 1):
+        raise ValueError(f"{name} must be a contiguous 1-D tensor")
+    if t.numel() < min_len:
+        raise ValueError(f"{name} has {t.numel()} elements, needs at least {min_len}")
+    if dtypes is not None and t.dtype not in dtypes:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected one of {sorted(str(d) for d in dtypes)}")
+
+
+def _check_devices(plan, *tensors):
+    devs = {t.device for t in tensors if t is not None}
+    if len(devs) > 1:
+        raise ValueError(f"tensors are on different devices: {sorted(str(d) for d in devs)}")
+    if plan.device is None:
+        raise ValueError("host-only plan (device=False): launches need a device plan")
+    for d in devs:
+        if d.type != "cuda" or (d.index if d.index is not None else torch.cuda.current_device()) != plan.device:
+            raise ValueError(f"tensor on {d}, the plan lives on cuda:{plan.device}")
+
+
+_F16ISH = (torch.float16, torch.bfloat16)
+_VALUE_DTYPES = tuple(_TORCH_DTYPE)
+
+
 class Plan:
     """Panel tiling of a CSR pattern (host tables + device copies).
 
@@ -75,6 +121,8 @@ class Plan:
         self.device_bytes = info.device_bytes
         self.build_ms = info.build_ms
         self.dense_elems = info.dense_elems
+        # the GPU holding the device tables (None for a host-only plan)
+        self.device = None if info.device_bytes == 0 else torch.cuda.current_device()
 
     @property
     def handle(self):
@@ -214,14 +262,25 @@ def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, s
     dense=True: a_tiled holds the plan's panel-dense tiles (Plan.pack_dense);
     the result is bit-identical to the tiled-values launch.
     """
+    _check_matrix(B, "B", rows=plan.n_cols, dtypes=_VALUE_DTYPES)
     K, N = B.shape
-    if K != plan.n_cols:
-        raise ValueError(f"B has {K} rows, plan has {plan.n_cols} columns")
     if out is None:
         out = torch.empty((plan.n_rows, N), dtype=torch.float32, device=B.device)
+    _check_matrix(out, "out", dtypes=(torch.float32,), min_rows=plan.n_rows, min_cols=N)
+    if dense:
+        if a_tiled is None or a_tiled.numel() != plan.dense_elems:
+            raise ValueError(f"dense A must hold plan.dense_elems = {plan.dense_elems} elements")
+        _check_vector(a_tiled, "dense A", plan.dense_elems, _VALUE_DTYPES)
+    elif plan.nnz:
+        if a_tiled is None:
+            raise ValueError("a_tiled is required for a plan with nonzeros")
+        _check_vector(a_tiled, "a_tiled", plan.nnz, _VALUE_DTYPES)
+    if row_scale is not None:
+        _check_vector(row_scale, "row_scale", plan.n_rows, (torch.float32,))
+    if col_scale is not None:
+        _check_vector(col_scale, "col_scale", N, (torch.float32,))
+    _check_devices(plan, a_tiled, B, out, row_scale, col_scale)
     at = tca_dtype(a_tiled) if a_tiled is not None else F16
-    if dense and (a_tiled is None or a_tiled.numel() != plan.dense_elems):
-        raise ValueError(f"dense A must hold plan.dense_elems = {plan.dense_elems} elements")
     call("tca_spmm_dense" if dense else "tca_spmm", plan.handle, at, _ptr(a_tiled), tca_dtype(B), _ptr(B), B.stride(0), N,
          _ptr(out), out.stride(0), _ptr(row_scale), _ptr(col_scale), tile_n, _stream(stream))
     return out
@@ -229,11 +288,13 @@ def spmm(plan, a_tiled, B, out=None, row_scale=None, col_scale=None, tile_n=0, s
 
 def sddmm(plan, G, X, out=None, stream=None):
     """out_tiled[e] = sum_n G[row(e), n] X[col(e), n] (float32, tiled order)."""
-    M, N = G.shape
-    if M != plan.n_rows or X.shape[0] != plan.n_cols or X.shape[1] != N:
-        raise ValueError("sddmm: G must be M x N and X K x N")
+    _check_matrix(G, "G", rows=plan.n_rows, dtypes=_F16ISH + (torch.float32,))
+    N = G.shape[1]
+    _check_matrix(X, "X", rows=plan.n_cols, cols=N, dtypes=_F16ISH)
     if out is None:
         out = torch.empty(max(plan.nnz, 1), dtype=torch.float32, device=G.device)[:plan.nnz]
+    _check_vector(out, "out", plan.nnz, (torch.float32,))
+    _check_devices(plan, G, X, out)
     call("tca_sddmm", plan.handle, tca_dtype(G), _ptr(G), G.stride(0), tca_dtype(X), _ptr(X), X.stride(0), N,
          _ptr(out), _stream(stream))
     return out

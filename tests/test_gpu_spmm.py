@@ -243,8 +243,15 @@ def test_invalid_arguments_are_rejected(cuda):
         T.spmm(plan, t, dev(np.zeros((512, 256), np.uint8)))
     assert e.value.name == "TCA_ERR_UNSUPPORTED"
     host_only = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, 16, device=False)
-    with pytest.raises(T.TcaError):
-        T.spmm(host_only, t, B)
+    with pytest.raises(ValueError, match="host-only"):
+        T.spmm(host_only, t, B)                 # the wrapper refuses it ...
+    import ctypes
+    from tca_spmm import _lib
+    C = torch.empty((A.n_rows, 256), dtype=torch.float32, device=cuda)
+    with pytest.raises(T.TcaError) as e:        # ... and so does the C ABI
+        _lib.call("tca_spmm", host_only.handle, T.F16, ctypes.c_void_p(t.data_ptr()), T.F16,
+                  ctypes.c_void_p(B.data_ptr()), 256, 256, ctypes.c_void_p(C.data_ptr()), 256, None, None, 0, None)
+    assert e.value.name == "TCA_ERR_INVALID_ARG"
 
 
 def test_dropin_driver_bundled():
@@ -254,3 +261,41 @@ def test_dropin_driver_bundled():
     r = subprocess.run([exe, BUNDLED_SMTX], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "preprocess" in r.stdout and "our:" in r.stdout and "validation failed" not in r.stdout
+
+
+def test_bf16_full_size_standard_plans(cuda):
+    # 4096 x 4096 at 0.9, N = 512 in BF16 on the one-wave-per-group plans, tiled
+    # values and panel-dense tiles, against the FP64 oracle on the BF16 inputs
+    case = workloads.precision()
+    A = case.A
+    va = gen.f32_to_bf16_bits(A.values.view(np.float16).astype(np.float32))
+    vb = gen.f32_to_bf16_bits(case.B.view(np.float16).astype(np.float32))
+    ref = O.spmm_f64(A.n_rows, case.N, A.row_offset, A.col_idx, O.bf16_bits_to_f32(va).astype(np.float64),
+                     O.bf16_bits_to_f32(vb).astype(np.float64))
+    B = dev_bf16(vb)
+    for rpw in RPWS:
+        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rpw)
+        t = plan.pack(dev_bf16(va))
+        assert rel_l2(T.spmm(plan, t, B, tile_n=64).cpu().numpy(), ref) <= 1e-5, rpw
+        assert rel_l2(T.spmm(plan, plan.pack_dense(t), B, tile_n=64, dense=True).cpu().numpy(), ref) <= 1e-5, rpw
+
+
+def test_fp8_full_size_standard_plans(cuda):
+    # 4096 x 4096 at 0.9, N = 512 in E4M3 with per-row / per-column scales,
+    # against the FP64 oracle on the quantised inputs and scales
+    case = workloads.precision()
+    A = case.A
+    q, s = T.quantize_rows_e4m3(dev(A.row_offset), dev_f16(A.values))
+    qb, sb = T.quantize_cols_e4m3(dev_f16(case.B))
+    qa = O.e4m3_to_f32(q.cpu().numpy()).astype(np.float64)
+    qbf = O.e4m3_to_f32(qb.cpu().numpy()).astype(np.float64)
+    ref = O.spmm_f64(A.n_rows, case.N, A.row_offset, A.col_idx, qa, qbf)
+    ref *= s.cpu().numpy().astype(np.float64)[:, None] * sb.cpu().numpy().astype(np.float64)[None, :]
+    for rpw in RPWS:
+        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rpw)
+        t = plan.pack(q)
+        C = T.spmm(plan, t, qb, row_scale=s, col_scale=sb, tile_n=64).cpu().numpy()
+        assert rel_l2(C, ref) <= 1e-5, rpw
+        Cd = T.spmm(plan, plan.pack_dense(t), qb, row_scale=s, col_scale=sb, tile_n=64, dense=True).cpu().numpy()
+        assert rel_l2(Cd, ref) <= 1e-5, rpw
+    assert rel_l2(C, oracle_f64(A, case.B)) < 0.1                # format error of E4M3 inputs

@@ -112,3 +112,26 @@ def test_gpu_gradients_dense_tiles_match_tiled(cuda, problem):
         out.append((Y.detach(), layer.weight.grad, X.grad))
     for a, b in zip(*out):
         assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_fp32_activations(cuda, problem):
+    """An FP32 X: the forward rounds it to FP16 on load and dW = SDDMM(G, X)
+    takes the same rounded X; W.grad and X.grad against the FP64 formula on the
+    FP16-rounded X (rel-L2 <= 1e-3, G rounded to FP16 by the kernels)."""
+    p = problem
+    W = p["W"]
+    rng = np.random.default_rng(5)
+    X32 = (p["X"].view(np.float16).astype(np.float32) + rng.standard_normal(p["X"].shape).astype(np.float32) * 1e-3)
+    x64 = X32.astype(np.float16).astype(np.float64)
+    layer = T.SparseLinear(p["M"], p["K"], W.row_offset, W.col_idx, W.values, rows_per_wave=16)
+    X = dev(X32).requires_grad_(True)
+    Y = layer(X)
+    Yref = O.spmm_f64(p["M"], p["N"], W.row_offset, W.col_idx, p["w64"], x64)
+    assert rel_l2(Y.detach().cpu().numpy(), Yref) <= 1e-5
+    (Y * dev(p["G"])).sum().backward()
+    assert X.grad.dtype == torch.float32
+    ref_dw = O.sddmm_f64(p["M"], p["N"], W.row_offset, W.col_idx, p["g64"], x64)
+    dW = layer.csr_values(layer.weight.grad).float().cpu().numpy()
+    assert rel_l2(dW, ref_dw) <= 1e-3
+    assert rel_l2(X.grad.cpu().numpy(), p["dX"]) <= 1e-3

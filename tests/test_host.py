@@ -206,6 +206,10 @@ def test_plan_rejects_bad_csr():
     with pytest.raises(T.TcaError) as e:
         T.Plan(2, 4, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), rows_per_wave=8, device=False)
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+    # an offset past nnz ahead of a decrease: rejected before any column is read
+    with pytest.raises(T.TcaError) as e:
+        T.Plan(2, 4, np.array([0, 100, 5], np.int32), np.array([0, 1, 2, 3, 1], np.int32), device=False)
+    assert e.value.name == "TCA_ERR_FORMAT"
 
 
 # ------------------------------------------------------------ MatrixMarket --
