@@ -208,6 +208,20 @@ template <int NT> constexpr int tr_pitch() { return NT == 16 ? 48 : NT + 16; }
 // keeps its size.
 __device__ __forceinline__ int tr_row(int k, int pitch_bytes) { return (k ^ ((k >> 1) & 4)) * pitch_bytes; }
 
+// Row of the m-th row slot when a workgroup writes a B tile in 16-byte pieces
+// of SEGS per row (piece p: row slot p / SEGS, segment p % SEGS).  A write
+// cycle serves 16 lanes, i.e. 16 / SEGS rows; taken in order, consecutive rows
+// of these pitches overlap on 64 banks, so the low three bits of m are rotated
+// to spread each cycle's rows four (two rows per cycle) or two (four rows)
+// apart, onto distinct banks.  A permutation of the rows.
+template <int SEGS> __device__ __forceinline__ int piece_row(int m)
+{
+    constexpr int S = SEGS == 8 ? 1 : SEGS == 4 ? 2 : 0;
+    if constexpr (S == 0) return m;
+    const int low = m & 7;
+    return (m & ~7) | (((low >> S) | (low << (3 - S))) & 7);
+}
+
 // Orders one wave's LDS accesses (write -> read -> write of its own tiles).
 // LDS instructions of a wave execute in issue order, so only the compiler
 // has to be kept from moving memory operations across this point.

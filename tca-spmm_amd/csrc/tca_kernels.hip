@@ -576,7 +576,7 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
     for (int j = 0; j < L::PPT; j++) {
         const int pc = tid + j * WM * 64;
         pvalid[j] = pc < L::PIECES;
-        pk[j] = pvalid[j] ? pc / L::SEGS : 0;
+        pk[j] = pvalid[j] ? piece_row<L::SEGS>(pc / L::SEGS) : 0;
         pseg[j] = pvalid[j] ? pc % L::SEGS : 0;
     }
     const int seg_lane = lane <= PPW ? lane : PPW;
@@ -836,8 +836,8 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const int pc = tid + j * WM * 64;
-        pk[j] = pc / L::SEGS;
-        const int seg = pc - pk[j] * L::SEGS;
+        pk[j] = piece_row<L::SEGS>(pc / L::SEGS);
+        const int seg = pc % L::SEGS;
         const int ncol = n0 + 8 * seg;
         pok[j] = ncol < p.N;
         pcol[j] = (uint32_t)ncol * RB::BES;
@@ -1101,11 +1101,14 @@ struct SddmmParams {
 // writing out[dst[e]] (dW in the caller's plan's tiled order).
 constexpr int kShRows = 64;           // rows per group of the SDDMM tiling
 constexpr int kShPitch = 40;          // X tile row pitch (elements): 32 n + 8
+// Row pitch (floats) of a wave's staged 16 x 32 result tile: the four lane
+// groups' rows land 16 banks apart on a 64-bank LDS (and 16 apart on 32 banks)
+constexpr int kShStage = 36;
 
 template <int CB>
 struct ShLds {
     static constexpr int XT = CB * 32 * kShPitch * 2;           // one X tile (bytes)
-    static constexpr int STAGE = 16 * 33 * 4;                   // one wave's result tile
+    static constexpr int STAGE = 16 * kShStage * 4;             // one wave's result tile
     static constexpr int BYTES = 2 * XT + 4 * STAGE;
     static constexpr int PIECES = CB * 32 * 4;                  // 16-byte X pieces per step
     static constexpr int XP = (PIECES + 255) / 256;             // X pieces per thread and step
@@ -1138,7 +1141,10 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     bool xval[L::XP];
 #pragma unroll
     for (int j = 0; j < L::XP; j++) {
-        const int pc = tid + 256 * j, row = pc >> 2, q = pc & 3, b = row >> 5;
+        // four 16-byte pieces per X row; a write cycle's 16 lanes take rows
+        // four apart (distinct banks at this pitch), every row once
+        const int pc = tid + 256 * j, m = pc >> 2, q = pc & 3;
+        const int row = (m & ~15) | ((m & 3) << 2) | ((m >> 2) & 3), b = row >> 5;
         xval[j] = pc < L::PIECES;
         const int col = xval[j] && b < nc ? gbl(p.chunk_cols)[(long)(c0 + b) * 32 + (row & 31)] : -1;
         xsrc[j] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + 8u * q) * 2u : kOOB;
@@ -1245,18 +1251,18 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * 33 + 16 * h + (lane & 15)] = acc[b][h][j];
+            for (int j = 0; j < 4; j++) tile[(4 * (lane >> 4) + j) * kShStage + 16 * h + (lane & 15)] = acc[b][h][j];
         wave_lds_sync();
 #pragma unroll
         for (int k = 0; k < kShEP; k++) {
             if (e0[b] + k * 64 + lane < e1[b]) {
                 const uint32_t sl = slot[b][k];
-                gbl(p.out)[dsti[b][k]] = tile[(sl >> 5) * 33 + (sl & 31)];
+                gbl(p.out)[dsti[b][k]] = tile[(sl >> 5) * kShStage + (sl & 31)];
             }
         }
         for (int e = e0[b] + kShEP * 64 + lane; e < e1[b]; e += 64) {
             const int sl = gbl(p.entry_slot)[e];
-            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(sl >> 5) * 33 + (sl & 31)];
+            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(sl >> 5) * kShStage + (sl & 31)];
         }
         wave_lds_sync();
     }
