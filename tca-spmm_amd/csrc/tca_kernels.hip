@@ -385,35 +385,53 @@ __device__ __forceinline__ void spmm_unit(const SpmmParams &p, int g, int tile, 
     wave_lds_sync();  // this wave's LDS tiles are clean for its next use
 }
 
-// Direct FP32 stores (or atomic adds) of one wave's accumulators: lane holds
-// D[4*(lane/16)+j][lane%16] of each 16x16 tile.
+// FP32 stores (or atomic adds) of one wave's accumulator tiles: acc[i][s] is
+// the 16x16 tile of rows row0 + 16i .. and columns n0 + 16s .., its lane
+// holding D[4*(lane/16)+j][lane%16].  The uniform flags pick one of three loops
+// (plain stores; scaled stores; scaled atomic adds) so that no element pays a
+// branch, or a scale load, that it does not need.
+template <int NI, int NS>
+__device__ __forceinline__ void store_tiles(const SpmmParams &p, int row0, int n0, const f32x4 (&acc)[NI][NS],
+                                            bool atomic)
+{
+    const int lane = lane_id();
+    const int rbase = row0 + 4 * (lane >> 4);
+    auto run = [&](auto scaled, auto add) {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = rbase + 16 * i + j;
+                if (row >= p.n_rows) continue;
+                float rs = 1.0f;
+                if constexpr (decltype(scaled)::value) rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
+                float *crow = p.C + (long)row * p.ldc;
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int n = n0 + 16 * s + (lane & 15);
+                    if (n < p.N) {
+                        float v = acc[i][s][j];
+                        if constexpr (decltype(scaled)::value) {
+                            v *= rs;
+                            if (p.col_scale) v *= gbl(p.col_scale)[n];
+                        }
+                        if constexpr (decltype(add)::value) atomic_add_f32(crow + n, v);
+                        else gbl(crow)[n] = v;
+                    }
+                }
+            }
+        }
+    };
+    if (atomic) run(std::true_type{}, std::true_type{});
+    else if (p.row_scale || p.col_scale) run(std::true_type{}, std::false_type{});
+    else run(std::false_type{}, std::false_type{});
+}
+
 template <int RPW, int NT>
 __device__ __forceinline__ void spmm_store_direct(const SpmmParams &p, int g, int tile,
                                                   const f32x4 (&acc)[RPW / 16][NT / 16], bool atomic)
 {
-    const int lane = lane_id();
-    const int n0 = tile * NT;
-    const int rbase = g * RPW + 4 * (lane >> 4);
-#pragma unroll
-    for (int i = 0; i < RPW / 16; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = rbase + 16 * i + j;
-            if (row >= p.n_rows) continue;
-            const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
-            float *crow = p.C + (long)row * p.ldc;
-#pragma unroll
-            for (int s = 0; s < NT / 16; s++) {
-                const int n = n0 + 16 * s + (lane & 15);
-                if (n < p.N) {
-                    float v = acc[i][s][j] * rs;
-                    if (p.col_scale) v *= gbl(p.col_scale)[n];
-                    if (atomic) atomic_add_f32(crow + n, v);
-                    else gbl(crow)[n] = v;
-                }
-            }
-        }
-    }
+    store_tiles(p, g * RPW, tile * NT, acc, atomic);
 }
 
 // LDS bytes one wave of spmm_kernel needs: its B and A tiles during the chunk
@@ -751,28 +769,8 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
         }
     }
 
-    // D[4*(lane/16)+j][lane%16] of each 16x16 tile -> C (atomic when the group is split)
-    const int rbase = g * RPG + w * PPW * 16 + 4 * (lane >> 4);
-#pragma unroll
-    for (int i = 0; i < PPW; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = rbase + 16 * i + j;
-            if (row >= p.n_rows) continue;
-            const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
-            float *crow = p.C + (long)row * p.ldc;
-#pragma unroll
-            for (int s = 0; s < NSUB; s++) {
-                const int n = n0 + 16 * s + (lane & 15);
-                if (n < p.N) {
-                    float v = acc[i][s][j] * rs;
-                    if (p.col_scale) v *= gbl(p.col_scale)[n];
-                    if (p.split) atomic_add_f32(crow + n, v);
-                    else gbl(crow)[n] = v;
-                }
-            }
-        }
-    }
+    // -> C (atomic when the group is split)
+    store_tiles(p, g * RPG + w * PPW * 16, n0, acc, p.split != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -817,6 +815,7 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
     constexpr int PPG = WM * PPW;                       // panels of the plan's group
     constexpr int RED = WM * NSUB * 1024;               // one panel of a K group's waves (bytes)
     static_assert(Elt<AT>::ES == 2, "block units take 16-bit A");
+    static_assert(D % 2 == 0, "the LDS buffer of a step is its register stage's parity");
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int h = wv / WM, w = wv - h * WM;
     const int tid = w * 64 + lane;
@@ -853,64 +852,109 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
 #pragma unroll
         for (int s = 0; s < NSUB; s++) acc[i][s] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    struct Stage {
-        typename RB::Raw rows[PPT];
-        Oct aop[PPW];
+    // Software pipeline over the K group's chunks, one barrier per step.  At
+    // step k (chunk c_lo + k): the barrier; the B pieces of chunk k+2 go to LDS
+    // buffer k&1 (its chunk k was read out at step k-1); the MFMA operands of
+    // chunk k+1 are read from buffer (k+1)&1 (written at step k-1) into
+    // registers; the MFMAs of chunk k run on operands read at step k-1, so they
+    // overlap this step's LDS traffic and the next barrier.  Global loads run D
+    // steps ahead: B pieces of chunk k+2+D, A tiles of chunk k+D, column
+    // indices of chunk k+2+2D.
+    struct BFrag {
+        Oct b[NSUB];
     };
-    int ix[D][PPT];
-    Stage st[D];
+    typename RB::Raw pb[D][PPT];   // B pieces, chunk k+2 at step k (k % D)
+    Oct pa[D][PPW];                // A tiles, chunk k at step k
+    int ix[D][PPT];                // column indices, chunk k+2+D at step k
+    BFrag fr[2];                   // MFMA B operands, chunk k at step k (k & 1)
     auto load_index = [&](int c, int (&x)[PPT]) {
 #pragma unroll
         for (int j = 0; j < PPT; j++) x[j] = (int)buf_b32(r_cols, ((uint32_t)c * 32u + (uint32_t)pk[j]) * 4u);
     };
-    auto load_data = [&](int c, const int (&x)[PPT], Stage &sg) {
+    auto load_b = [&](const int (&x)[PPT], typename RB::Raw (&rows)[PPT]) {
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
             const bool ok = x[j] >= 0 && pok[j];
-            sg.rows[j] = RB::load(r_b, ok ? (uint32_t)x[j] * ldb_bytes + pcol[j] : kOOB);
+            rows[j] = RB::load(r_b, ok ? (uint32_t)x[j] * ldb_bytes + pcol[j] : kOOB);
         }
+    };
+    auto load_a = [&](int c, Oct (&aop)[PPW]) {
 #pragma unroll
         for (int i = 0; i < PPW; i++)
-            sg.aop[i] = buf_b128(r_a, (uint32_t)c * (uint32_t)(PPG * 1024) + a_lane + (uint32_t)i * 1024u);
+            aop[i] = buf_b128(r_a, (uint32_t)c * (uint32_t)(PPG * 1024) + a_lane + (uint32_t)i * 1024u);
     };
-    // Every K group runs `iters` barrier steps (the shorter ones idle in their
-    // last); loads past a range re-read its last chunk (or, for an empty range,
-    // its first, always a chunk of the plan or out of range for the buffer
+    auto put_b = [&](uint8_t *buf, const typename RB::Raw (&rows)[PPT]) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) *reinterpret_cast<Oct *>(buf + plds[j]) = RL::cvt(rows[j]);
+    };
+    auto read_frag = [&](const uint8_t *buf, BFrag &f) {
+#pragma unroll
+        for (int sb = 0; sb < NSUB; sb++) {
+            const uint2 lo = lds_read_tr16(buf, boff_lo + 32 * sb);
+            const uint2 hi = lds_read_tr16(buf, boff_hi + 32 * sb);
+            f.b[sb] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+    };
+    // Every K group runs `iters` steps (the shorter ones idle in their last);
+    // loads past a range re-read its last chunk (or, for an empty range, its
+    // first, always a chunk of the plan or out of range for the buffer
     // descriptors, which read zeros), so every load is unconditional.
     const int c_last = max(c_hi - 1, c_lo);
+    auto cc = [&](int k) { return min(c_lo + k, c_last); };
+    {
+        int x0[PPT], x1[PPT];
+        typename RB::Raw b0[PPT], b1[PPT];
+        load_index(cc(0), x0);
+        load_index(cc(1), x1);
 #pragma unroll
-    for (int s = 0; s < D; s++) load_index(min(c_lo + s, c_last), ix[s]);
-#pragma unroll
-    for (int s = 0; s < D; s++) {
-        load_data(min(c_lo + s, c_last), ix[s], st[s]);
-        load_index(min(c_lo + s + D, c_last), ix[s]);
-    }
-    vm_drain();
-    for (int k0 = 0; k0 < iters; k0 += D) {
+        for (int s = 0; s < D; s++) load_index(cc(s + 2), ix[s]);
+        load_b(x0, b0);
+        load_b(x1, b1);
+        // the stages' loads in the loop's own order (B, A, index per stage), so
+        // that the waits the compiler derives in the loop from its entry and
+        // back-edge states together stay those of the steady state
 #pragma unroll
         for (int s = 0; s < D; s++) {
-            const int k = k0 + s;
-            const int c = c_lo + k;
-            if (k < iters) {
-                uint8_t *buf = smem + (k & 1) * L::BTILE;
-#pragma unroll
-                for (int j = 0; j < PPT; j++) *reinterpret_cast<Oct *>(buf + plds[j]) = RL::cvt(st[s].rows[j]);
-                __syncthreads();
-                if (c < c_hi) {
-#pragma unroll
-                    for (int sb = 0; sb < NSUB; sb++) {
-                        const uint2 lo = lds_read_tr16(buf, boff_lo + 32 * sb);
-                        const uint2 hi = lds_read_tr16(buf, boff_hi + 32 * sb);
-                        const Oct b = make_uint4(lo.x, lo.y, hi.x, hi.y);
-#pragma unroll
-                        for (int i = 0; i < PPW; i++) acc[i][sb] = Elt<AT>::mma(st[s].aop[i], b, acc[i][sb]);
-                    }
-                }
-            }
-            load_data(min(c + D, c_last), ix[s], st[s]);
-            load_index(min(c + 2 * D, c_last), ix[s]);
+            __builtin_amdgcn_sched_barrier(0);
+            load_b(ix[s], pb[s]);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(cc(s), pa[s]);
+            __builtin_amdgcn_sched_barrier(0);
+            load_index(cc(s + 2 + D), ix[s]);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        put_b(smem, b0);
+        put_b(smem + L::BTILE, b1);
+        __syncthreads();
+        read_frag(smem, fr[0]);
     }
+    auto step = [&](int k, int s) {   // s = k % D, a compile-time register stage
+        const int fb = s & 1;             // D is even: k & 1 == s & 1
+        __syncthreads();
+        put_b(smem + fb * L::BTILE, pb[s]);
+        read_frag(smem + (1 - fb) * L::BTILE, fr[1 - fb]);
+        if (c_lo + k < c_hi) {
+#pragma unroll
+            for (int sb = 0; sb < NSUB; sb++)
+#pragma unroll
+                for (int i = 0; i < PPW; i++) acc[i][sb] = Elt<AT>::mma(pa[s][i], fr[fb].b[sb], acc[i][sb]);
+        }
+        load_b(ix[s], pb[s]);
+        load_a(cc(k + D), pa[s]);
+        load_index(cc(k + 2 + 2 * D), ix[s]);
+    };
+    // whole rounds of D steps with no guard inside (a path that skipped a
+    // step's loads would reach the loop head with fewer loads in flight, and
+    // the compiler would size every iteration's waits for that path), then the
+    // last partial round
+    int k0 = 0;
+    for (; k0 + D <= iters; k0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; s++) step(k0 + s, s);
+    }
+#pragma unroll
+    for (int s = 0; s < D; s++)
+        if (k0 + s < iters) step(k0 + s, s);
     if constexpr (KS > 1) {
         // K groups 1..KS-1 hand their partial tiles to group 0, one panel at a
         // time (the sum runs in K-group order: deterministic)
@@ -932,30 +976,7 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
             }
         }
     }
-    if (h == 0) {
-        // D[4*(lane/16)+j][lane%16] of each 16x16 tile -> C
-        const int rbase = g * (PPG * 16) + w * PPW * 16 + 4 * (lane >> 4);
-#pragma unroll
-        for (int i = 0; i < PPW; i++) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int row = rbase + 16 * i + j;
-                if (row >= p.n_rows) continue;
-                const float rs = p.row_scale ? gbl(p.row_scale)[row] : 1.0f;
-                float *crow = p.C + (long)row * p.ldc;
-#pragma unroll
-                for (int s = 0; s < NSUB; s++) {
-                    const int n = n0 + 16 * s + (lane & 15);
-                    if (n < p.N) {
-                        float v = acc[i][s][j] * rs;
-                        if (p.col_scale) v *= gbl(p.col_scale)[n];
-                        if (atomic) atomic_add_f32(crow + n, v);
-                        else gbl(crow)[n] = v;
-                    }
-                }
-            }
-        }
-    }
+    if (h == 0) store_tiles(p, g * (PPG * 16) + w * PPW * 16, n0, acc, atomic);
     __syncthreads();   // LDS is clean for the workgroup's next unit
 }
 
