@@ -355,13 +355,15 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     for (auto &un : units)
         if (item_split[un.item]) un.atomic = 1;
 
-    // Partitions (one per XCD): units of one (item, group) -- its N-tiles,
+    // Partitions (one per XCD): the units of one (item, group) -- its N-tiles,
     // chunk pieces or SDDMM blocks, which read the same A tiles or G rows --
-    // go to one partition in bundles of up to kMaxBundle, so its L2 fetches
-    // them once per bundle (larger bundles balance the partitions worse).
-    // Bundles are assigned heaviest first to the least-loaded partition; inside
-    // a partition the units run heaviest first.
-    constexpr size_t kMaxBundle = 2;
+    // form a bundle; bundles in item order (a layer's forward, dX and dW
+    // items are adjacent, sharing X and G), then group order, are cut into
+    // kBatchParts runs of equal estimated cost, so that an XCD's L2 fetches a
+    // group's tiles once and an item's B operand once per XCD its run spans.
+    // Inside a partition the units run heaviest first; workgroups whose
+    // partition is exhausted take units from the others (the kernel), which
+    // absorbs the cost model's errors at the tail.
     const int grid = std::max(1, std::min((int)units.size(), workers));
     const int nparts = std::min(tca::kBatchParts, grid);
     std::vector<int> order(units.size());
@@ -369,24 +371,23 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return units[a].item != units[b].item ? units[a].item < units[b].item : units[a].g < units[b].g;
     });
-    struct Bundle { size_t lo, hi; double cost; };
-    std::vector<Bundle> bundles;
-    for (size_t k = 0; k < order.size();) {
-        size_t e = k;
-        double c = 0.0;
-        while (e < order.size() && e - k < kMaxBundle && units[order[e]].item == units[order[k]].item &&
-               units[order[e]].g == units[order[k]].g)
-            c += cost[order[e++]];
-        bundles.push_back({k, e, c});
-        k = e;
-    }
-    std::stable_sort(bundles.begin(), bundles.end(), [](const Bundle &a, const Bundle &b) { return a.cost > b.cost; });
     std::vector<std::vector<int>> part(nparts);
-    std::vector<double> load(nparts, 0.0);
-    for (const Bundle &bd : bundles) {
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[x] += bd.cost;
-        for (size_t k = bd.lo; k < bd.hi; k++) part[x].push_back(order[k]);
+    {
+        double total = 0.0;
+        for (double c : cost) total += c;
+        const double target = std::max(total / nparts, 1e-9);
+        double cum = 0.0;
+        for (size_t k = 0; k < order.size();) {
+            size_t e = k;
+            double c = 0.0;
+            while (e < order.size() && units[order[e]].item == units[order[k]].item &&
+                   units[order[e]].g == units[order[k]].g)
+                c += cost[order[e++]];
+            const int x = std::min(nparts - 1, (int)((cum + 0.5 * c) / target));
+            for (size_t q = k; q < e; q++) part[x].push_back(order[q]);
+            cum += c;
+            k = e;
+        }
     }
     std::vector<tca::BatchUnitDev> sorted;
     std::vector<int> part_off(tca::kBatchParts + 1, 0);

@@ -1567,23 +1567,39 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
     // first unit is static (its rank r in the partition); later ones come from
     // the partition's counter, which hands out ranks gx + 0, 1, ....  One
     // atomic address per partition keeps the claims from queueing on a single
-    // line.  (P <= gridDim.x, so every partition has workgroups.)
+    // line.  A workgroup whose partition is exhausted moves on to the next
+    // ones in turn (x+1, x+2, ... mod P) and claims from their counters, so
+    // that no XCD idles while another still has units; it ends when every
+    // partition is exhausted.  (P <= gridDim.x, so every partition has
+    // workgroups.)
     (void)n_units;
     const int P = nparts;
     const int x = (int)blockIdx.x % P, r = (int)blockIdx.x / P;
-    const int gx = ((int)gridDim.x - x + P - 1) / P;
-    const int p_lo = part_off[x], p_n = part_off[x + 1] - p_lo;
-    int *cnt = counters + x * kCounterStride;
+    int xs = x, hops = 0;   // thread 0: the partition it claims from, partitions left
     for (bool first = true;; first = false) {
-        int k = r;
-        if (!first) {
-            if (threadIdx.x == 0) next_unit = gx + atomicAdd(cnt, 1);
+        int u;
+        if (first && r < part_off[x + 1] - part_off[x]) {
+            u = part_off[x] + r;
+        } else {
+            if (threadIdx.x == 0) {
+                int found = -1;
+                while (hops < P) {
+                    const int gxs = ((int)gridDim.x - xs + P - 1) / P;
+                    const int k = gxs + atomicAdd(counters + xs * kCounterStride, 1);
+                    if (k < part_off[xs + 1] - part_off[xs]) {
+                        found = part_off[xs] + k;
+                        break;
+                    }
+                    xs = xs + 1 == P ? 0 : xs + 1;
+                    hops++;
+                }
+                next_unit = found;
+            }
             __syncthreads();
-            k = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
+            u = __builtin_amdgcn_readfirstlane(next_unit);   // uniform: descriptors load into SGPRs
             __syncthreads();                                  // all have read next_unit
         }
-        if (k >= p_n) break;
-        const int u = p_lo + k;
+        if (u < 0) break;
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
