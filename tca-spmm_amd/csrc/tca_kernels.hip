@@ -1120,15 +1120,24 @@ struct SddmmParams {
 // fetched once per 64 rows instead of once per 16.  Each wave then stages its
 // 16x32 result tiles in its own LDS area and samples them at the entry slots,
 // writing out[dst[e]] (dW in the caller's plan's tiled order).
+//
+// X tile layout: 16-byte pieces, a 1 KB block per 16 tile rows; octet q (n =
+// 8q .. 8q+7) of row 16*rb + r sits in slot 64*rb + 16q + (r ^ 2q).  Measured
+// on gfx950's LDS (bank-conflict counters over lane patterns): the MFMA
+// operand reads (lane 16q + r) and the piece writes (four lanes per X row, so
+// that the global loads stay 64-byte coalesced) are both conflict-free with
+// this XOR, while padded row pitches, rotations and XORs of the higher slot
+// bits conflict on one side or the other.
+__device__ __forceinline__ int sh_slot(int r, int q) { return 16 * q + (r ^ (2 * q)); }
+
 constexpr int kShRows = 64;           // rows per group of the SDDMM tiling
-constexpr int kShPitch = 40;          // X tile row pitch (elements): 32 n + 8
 // Row pitch (floats) of a wave's staged 16 x 32 result tile: the four lane
 // groups' rows land 16 banks apart on a 64-bank LDS (and 16 apart on 32 banks)
 constexpr int kShStage = 36;
 
 template <int CB>
 struct ShLds {
-    static constexpr int XT = CB * 32 * kShPitch * 2;           // one X tile (bytes)
+    static constexpr int XT = CB * 32 * 64;                     // one X tile (bytes)
     static constexpr int STAGE = 16 * kShStage * 4;             // one wave's result tile
     static constexpr int BYTES = 2 * XT + 4 * STAGE;
     static constexpr int PIECES = CB * 32 * 4;                  // 16-byte X pieces per step
@@ -1162,19 +1171,18 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
     bool xval[L::XP];
 #pragma unroll
     for (int j = 0; j < L::XP; j++) {
-        // four 16-byte pieces per X row; a write cycle's 16 lanes take rows
-        // four apart (distinct banks at this pitch), every row once
-        const int pc = tid + 256 * j, m = pc >> 2, q = pc & 3;
-        const int row = (m & ~15) | ((m & 3) << 2) | ((m >> 2) & 3), b = row >> 5;
+        const int pc = tid + 256 * j, q = pc & 3, r = (pc >> 2) & 15;
+        const int row = 16 * (pc >> 6) + r, b = row >> 5;
         xval[j] = pc < L::PIECES;
         const int col = xval[j] && b < nc ? gbl(p.chunk_cols)[(long)(c0 + b) * 32 + (row & 31)] : -1;
         xsrc[j] = col >= 0 ? ((uint32_t)col * (uint32_t)p.ldx + 8u * q) * 2u : kOOB;
-        xdst[j] = (row * kShPitch + 8 * q) * 2;
+        xdst[j] = 16 * (64 * (pc >> 6) + sh_slot(r, q));
     }
     const int grow = g * kShRows + 16 * w + (lane & 15);
     const uint32_t gsrc = grow < p.n_rows ? ((uint32_t)grow * (uint32_t)p.ldg + 8u * (lane >> 4)) * GB::BES : kOOB;
-    // B operand of (chunk b, half h): X tile row 32b + 16h + lane%16, octet lane/16
-    const int xread = ((lane & 15) * kShPitch + 8 * (lane >> 4)) * 2;
+    // B operand of (chunk b, half h): X tile row 32b + 16h + lane%16, octet
+    // lane/16 (block 2b + h)
+    const int xread = 16 * sh_slot(lane & 15, lane >> 4);
 
     // sampling indices of this wave's panel in each chunk, issued now so that
     // they arrive during the contraction (out-of-range entries read as zero
@@ -1244,7 +1252,7 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
             if (b < nc) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const Oct xb = *reinterpret_cast<const Oct *>(buf + xread + (32 * b + 16 * h) * kShPitch * 2);
+                    const Oct xb = *reinterpret_cast<const Oct *>(buf + xread + (2 * b + h) * 1024);
                     acc[b][h] = Elt<AT>::mma(ga, xb, acc[b][h]);
                 }
             }
