@@ -69,8 +69,8 @@ extern "C" tca_status tca_hipsparselt_spmm_24(int M, int N, int K, const uint16_
                                          "segmentation fault inside a process with only ROCm's hipSPARSELt loaded, "
                                          "cause not found yet");
     tca_status st = TCA_OK;
-    static hipsparseLtHandle_t handle;
-    static bool have_handle = false;
+    hipsparseLtHandle_t handle;
+    bool have_handle = false;
     hipsparseLtMatDescriptor_t mA, mB, mC;
     hipsparseLtMatmulDescriptor_t mm;
     hipsparseLtMatmulAlgSelection_t sel;
@@ -82,12 +82,13 @@ extern "C" tca_status tca_hipsparselt_spmm_24(int M, int N, int K, const uint16_
     size_t comp_size = 0, comp_buf_size = 0, ws_size = 0;
     const float alpha = 1.0f, beta = 0.0f;
     EventPair ev;
-    if (!have_handle) {
-        if (hipsparseLtInit(&handle) != HIPSPARSE_STATUS_SUCCESS) return fail(TCA_ERR_LIBRARY, "hipsparseLtInit failed");
-        have_handle = true;
-    }
+    if (hipsparseLtInit(&handle) != HIPSPARSE_STATUS_SUCCESS) return fail(TCA_ERR_LIBRARY, "hipsparseLtInit failed");
+    have_handle = true;
     if (!pruned) {
-        if (hipMalloc(&own_pruned, sizeof(uint16_t) * (size_t)M * K) != hipSuccess) return fail(TCA_ERR_ALLOC, "pruned A");
+        if (hipMalloc(&own_pruned, sizeof(uint16_t) * (size_t)M * K) != hipSuccess) {
+            st = fail(TCA_ERR_ALLOC, "pruned A");
+            goto done;
+        }
         pruned = own_pruned;
     }
     if (hipMalloc(&d_valid, sizeof(int)) != hipSuccess) { st = fail(TCA_ERR_ALLOC, "valid flag"); goto done; }
@@ -101,6 +102,7 @@ extern "C" tca_status tca_hipsparselt_spmm_24(int M, int N, int K, const uint16_
     TCA_SLT_TRY(hipsparseLtMatmulDescriptorInit(&handle, &mm, HIPSPARSE_OPERATION_NON_TRANSPOSE,
                                                 HIPSPARSE_OPERATION_NON_TRANSPOSE, &mA, &mB, &mC, &mC,
                                                 HIPSPARSELT_COMPUTE_32F));
+    // (the algorithm selection has no destroy call in this hipSPARSELt)
     TCA_SLT_TRY(hipsparseLtMatmulAlgSelectionInit(&handle, &sel, &mm, HIPSPARSELT_MATMUL_ALG_DEFAULT));
     TCA_SLT_TRY(hipsparseLtMatmulPlanInit(&handle, &plan, &mm, &sel));
     plan_ok = true;
@@ -144,5 +146,6 @@ done:
     if (dC_ok) hipsparseLtMatDescriptorDestroy(&mC);
     if (dB_ok) hipsparseLtMatDescriptorDestroy(&mB);
     if (dA_ok) hipsparseLtMatDescriptorDestroy(&mA);
+    if (have_handle) hipsparseLtDestroy(&handle);
     return st;
 }
