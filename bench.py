@@ -5,13 +5,14 @@ Headline workload (one "step"): the twelve-layer sparse-weight training step --
 twelve uneven CSR layers (4x 512x2048, 4x 2048x512, 2x 1024x1024,
 2x 4096x1024, sparsity U[0.5, 0.98], lognormal(0.5) rows, N in
 {128, 384, 1024}), FP16 weights and activations, FP32 upstream gradient G.
-A step is: the SGD update of every weight from the previous step's dW
-(w -= lr * dW, one elementwise launch over all layers' values), then forward
-Y = W X, dX = W^T G and dW = SDDMM(G, X^T) of all twelve layers.  The fused
-path runs it as the batch's prologue launch (FP16 copy of every G, panel-dense
-tiles of every W rebuilt from the updated values) + ONE persistent launch over
-all forward / dX / dW work units.  value = sum over layers of 3 * 2*nnz*N
-(forward, dX, dW) / step time, in TFLOP/s.
+A step is: forward Y = W X, dX = W^T G and dW = SDDMM(G, X^T) of all twelve
+layers with the current weights, and the SGD update of every weight from its
+dW (w -= lr * dW).  The fused path runs it as one batch run: the prologue
+launch (FP16 copy of every G, panel-dense tiles of every W rebuilt from the
+current values) + ONE persistent launch over all forward / dX / dW work
+units, whose SDDMM units also apply the weight update to every entry they
+write.  value = sum over layers of 3 * 2*nnz*N (forward, dX, dW) / step
+time, in TFLOP/s.
 
 Beside it, in the same run: the plain per-call FP16 path (36 tuned launches of
 tca_spmm / tca_sddmm on tiled values, with its own repack and the same SGD
@@ -99,6 +100,25 @@ def time_eager(torch, fn, iters, backlog=False):
     return s.elapsed_time(e) / iters
 
 
+def time_cold(torch, np, fn, reps, flush):
+    """Median ms of fn() with every cache level flushed first: a write of
+    `flush` (>= 512 MiB, beyond the L2s and the 256 MB Infinity Cache)
+    between runs, each run bracketed by its own event pair."""
+    fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    out = []
+    for _ in range(reps):
+        flush.fill_(1)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(st)
+        fn()
+        e.record(st)
+        torch.cuda.synchronize()
+        out.append(s.elapsed_time(e))
+    return float(np.median(out))
+
+
 def rel_l2(np, got, ref):
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)
@@ -111,11 +131,12 @@ def rel_l2(np, got, ref):
 class TrainStep:
     """The step run three ways, each path with the weights in its own
     preferred order, its own FP16 weight buffer and FP32 dW buffer (all layers
-    in one buffer each, so its SGD update is one launch):
+    in one buffer each, so a per-call path's SGD update is one launch):
 
       "f"  fused: weights in the tiled order of the batch's forward plan (the
            order its SDDMM writes dW in); the batch packs the forward and
-           transposed dense tiles from them in its prologue launch;
+           transposed dense tiles from them in its prologue launch, and its
+           SDDMM items update them from dW (sddmm_item(sgd=...));
       "t"  plain per call: weights in the tiled order of each layer's tuned
            forward plan; forward, dX (transposed plan reading the same values)
            and dW are 36 tca_spmm / tca_sddmm launches, no repack needed;
@@ -170,7 +191,7 @@ class TrainStep:
             f = L["f"]
             items += [T.spmm_item(f["plan"], f["w"], L["X"], L["Y"], f["tile"], dense="pack", values_order=f["plan"]),
                       T.spmm_item(f["plan_t"], f["w"], L["G"], L["dX"], f["tile_t"], dense="pack", values_order=f["plan"]),
-                      T.sddmm_item(f["plan"], L["G"], L["X"], f["dW"])]
+                      T.sddmm_item(f["plan"], L["G"], L["X"], f["dW"], sgd=(f["w"], SGD_LR))]
             self.layers.append(L)
             self.flops += 3 * 2 * W.nnz * N
             self.dense_flops += 3 * 2 * W.n_rows * W.n_cols * N
@@ -203,8 +224,7 @@ class TrainStep:
                 T.sddmm(P["plan"], L["G"], L["X"], out=P["dW"])
 
     def fused_step(self):
-        self.sgd("f")
-        self.batch.run()
+        self.batch.run()   # products with the current weights, then their update
 
     def per_call_step(self, path):
         def run():
@@ -292,8 +312,15 @@ def bench_train_step(torch, np, T, O, dev, args):
     ts.batch.run(stage=1)
     main_ms = time_eager(torch, lambda: ts.batch.run(stage=2), K, backlog=True)
     prologue_ms = time_eager(torch, lambda: ts.batch.run(stage=1), K, backlog=True)
+    # the separate update kernel the per-call paths launch (the fused path
+    # applies it inside the SDDMM units)
     sgd_ms = time_eager(torch, lambda: ts.sgd("f"), K, backlog=True)
     graph_ms = time_graph(torch, ts.fused_step, K)
+    # cold: every operand from HBM (a 512 MiB write evicts L2 and the Infinity Cache)
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
+    cold_ms = time_cold(torch, np, ts.fused_step, 10, flush)
+    plain_cold_ms = time_cold(torch, np, ts.per_call_step("t"), 10, flush)
+    del flush
     # the per-call paths, each with its own repack and the same SGD update
     plain_graph = time_graph(torch, ts.per_call_step("t"), 20)
     dense_graph = time_graph(torch, ts.per_call_step("d"), 20)
@@ -304,7 +331,8 @@ def bench_train_step(torch, np, T, O, dev, args):
     res = {
         "ms_per_step": ms_per_step, "wall_s_timed": wall, "flops_per_step": ts.flops,
         "dense_equivalent_flops_per_step": ts.dense_flops, "alg_bytes_per_step": ts.alg_bytes,
-        "main_launch_ms": main_ms, "prologue_ms": prologue_ms, "sgd_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
+        "main_launch_ms": main_ms, "prologue_ms": prologue_ms, "sgd_kernel_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
+        "cold_ms_per_step": cold_ms, "plain_per_call_cold_ms": plain_cold_ms,
         "plain_per_call_graph_ms": plain_graph, "plain_per_call_eager_ms": plain_eager,
         "dense_per_call_graph_ms": dense_graph, "dense_repack_ms": repack_d,
         "speedup_vs_plain_per_call": plain_graph / ms_per_step,
@@ -706,8 +734,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "twelve-layer sparse-weight training step: 4x 512x2048, 4x 2048x512, 2x 1024x1024, "
                                "2x 4096x1024 CSR layers, sparsity U[0.5,0.98], lognormal(0.5) rows, N in "
-                               "{128,384,1024}; SGD update + forward + dX + dW per step (fused: prologue launch "
-                               "packing dense tiles from the updated weights + one persistent launch)",
+                               "{128,384,1024}; forward + dX + dW + SGD update per step (fused: prologue launch "
+                               "packing dense tiles from the current weights + one persistent launch whose "
+                               "SDDMM units apply the update)",
                    "layers": len(ts.layers), "nnz_total": int(sum(L["case"].A.nnz for L in ts.layers)),
                    "flops_per_step": ts.flops, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -282,8 +282,10 @@ tca_status tca_quantize_cols_e4m3(int K, int N, const uint16_t *d_B, int ldb, ui
 /* Plain SGD step on weight values held in any order (tiled or CSR):
  * w[i] = round_to_nearest_even(w[i] - lr * g[i]) for i < n, w F16 or BF16, g FP32
  * in the same order (e.g. the dW a batch's SDDMM item writes).  w and g 16-byte
- * aligned.  No counterpart in the reference (it has no training); the
- * sparse-weight training step uses it so that the update is one launch. */
+ * aligned.  No counterpart in the reference (it has no training); a
+ * per-call training loop updates all layers' weights (one buffer) in one
+ * launch with it; a batch can instead fuse the update into its SDDMM items
+ * (tca_batch_item.sgd_w). */
 tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t n, float lr, void *stream);
 
 /* ------------------------------------------------------------------------ */
@@ -324,6 +326,17 @@ typedef struct tca_batch_item {
     int a_dense;           /* SPMM: 1 = `a` holds panel-dense tiles (tca_pack_dense);
                               2 = `a` holds values the batch packs every run (above) */
     const tca_plan *values_order;   /* a_dense = 2: order of `a` (NULL = CSR order) */
+    /* SDDMM items: optional SGD step fused into the sampling (no counterpart in
+       the reference; for training loops that would otherwise launch
+       tca_sgd_update after the batch).  When sgd_w is not NULL, every entry
+       the item writes to out[i] also updates sgd_w[i] <- round(fma(-sgd_lr,
+       out[i], sgd_w[i])) (sgd_w_dtype F16 or BF16, same order and length as
+       out), the same arithmetic as tca_sgd_update.  The item's own reads (G,
+       X) do not involve sgd_w, so the update may target weights whose dense
+       tiles the same batch packed in its prologue. */
+    void *sgd_w;
+    tca_dtype sgd_w_dtype;
+    float sgd_lr;
 } tca_batch_item;
 typedef struct tca_batch tca_batch;
 tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch);

@@ -229,6 +229,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         d.b = it.b;
         d.out = it.out;
         if (it.op == TCA_OP_SPMM) {
+            if (it.sgd_w)
+                return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SPMM item " + std::to_string(i) + ": sgd_w applies to SDDMM items");
             const int tile = it.tile_n ? it.tile_n : tca::default_tile_n(p, it.N);
             if (it.N <= 0 || it.N % 8 || it.ldb < it.N || it.ldc < it.N || it.ldb % 8 || it.ldc % 4 || !it.b || !it.out ||
                 !aligned16(it.b) || !aligned16(it.out) || (t.nnz > 0 && !it.a))
@@ -288,6 +290,15 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.entry_lds = aux->d_entry_lds;
             d.entry_src = aux->d_entry_src;
             d.dst = dst;
+            if (it.sgd_w) {
+                if (it.sgd_w_dtype != TCA_F16 && it.sgd_w_dtype != TCA_BF16)
+                    return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM item " + std::to_string(i) + ": sgd_w must be F16 or BF16");
+                if (!(it.sgd_lr == it.sgd_lr))
+                    return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM item " + std::to_string(i) + ": sgd_lr is NaN");
+                d.sgd_w = static_cast<uint16_t *>(it.sgd_w);
+                d.sgd_lr = it.sgd_lr;
+                d.sgd_bf16 = it.sgd_w_dtype == TCA_BF16 ? 1 : 0;
+            }
             d.rpw = ta.rpw;
             d.n_chunks = ta.n_chunks;
             d.ldb = it.ldb;   // X

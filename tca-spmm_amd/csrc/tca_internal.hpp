@@ -144,6 +144,9 @@ struct BatchItemDev {
     const int32_t *dst;          // SDDMM: output index of each entry (null: identity)
     const void *a, *b;
     float *out;
+    uint16_t *sgd_w;             // SDDMM: weights updated from each written entry (null: none)
+    float sgd_lr;
+    int32_t sgd_bf16;
 };
 struct BatchUnitDev {
     int32_t item, g, t;        // SPMM: (group, N-tile); SDDMM: (group, first chunk)

@@ -1108,7 +1108,29 @@ struct SddmmParams {
     uint32_t g_bytes, x_bytes;   // bytes of G and X from the first to the last used element
     const int32_t *dst;          // out index of each entry (null: the entry's own index)
     uint32_t nnz;                // entries of the tiling (entry_slot, dst)
+    uint16_t *sgd_w;             // batch items: weights updated from each written entry (null: none)
+    float sgd_lr;
+    int sgd_bf16;
 };
+
+// One SGD step on a 16-bit weight: round(fma(-lr, g, w)) -- the arithmetic of
+// sgd_kernel, so a fused update gives the bits of a separate tca_sgd_update.
+__device__ __forceinline__ uint32_t sgd_step(uint32_t bits, float g, float lr, bool bf16)
+{
+    const float x = bf16 ? __builtin_bit_cast(float, bits << 16) : (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
+    const float y = fmaf(-lr, g, x);
+    return bf16 ? f32_to_bf16_bits(y) : f32_to_f16_bits(y);
+}
+
+// out[d] = v, and the fused weight update when the item has one
+__device__ __forceinline__ void sddmm_emit(const SddmmParams &p, uint32_t d, float v)
+{
+    gbl(p.out)[d] = v;
+    if (p.sgd_w) {
+        auto w = gbl(p.sgd_w);
+        w[d] = (uint16_t)sgd_step(w[d], v, p.sgd_lr, p.sgd_bf16 != 0);
+    }
+}
 
 // SDDMM with shared X rows.  The plan here has 64-row groups: a workgroup of 4
 // waves owns (group g, chunks [c0, c0+CB)), wave w the group's panel w.  Per
@@ -1272,6 +1294,19 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
         step(1, s0 + 1);
     }
     if (s0 < steps) step(0, s0);
+    // fused weight update: the weights of the prefetched entries, all issued
+    // before the sampling so that their latency overlaps it
+    uint32_t wv[CB][kShEP];
+    if (p.sgd_w) {
+        const Rsrc r_w = make_rsrc(p.sgd_w, p.nnz * 2u);
+#pragma unroll
+        for (int b = 0; b < CB; b++)
+#pragma unroll
+            for (int k = 0; k < kShEP; k++) {
+                const bool ok = b < nc && e0[b] + k * 64 + lane < e1[b];
+                wv[b][k] = buf_b16(r_w, ok ? dsti[b][k] * 2u : kOOB);
+            }
+    }
     // sample this wave's panel: stage each chunk's 16x32 tile in the wave's area
     float *tile = reinterpret_cast<float *>(smem + 2 * L::XT + w * L::STAGE);
 #pragma unroll
@@ -1286,12 +1321,14 @@ __device__ __forceinline__ void sddmm_shared_unit(const SddmmParams &p, int g, i
         for (int k = 0; k < kShEP; k++) {
             if (e0[b] + k * 64 + lane < e1[b]) {
                 const uint32_t sl = slot[b][k];
-                gbl(p.out)[dsti[b][k]] = tile[(sl >> 5) * kShStage + (sl & 31)];
+                const float v = tile[(sl >> 5) * kShStage + (sl & 31)];
+                gbl(p.out)[dsti[b][k]] = v;
+                if (p.sgd_w) gbl(p.sgd_w)[dsti[b][k]] = (uint16_t)sgd_step(wv[b][k], v, p.sgd_lr, p.sgd_bf16 != 0);
             }
         }
         for (int e = e0[b] + kShEP * 64 + lane; e < e1[b]; e += 64) {
             const int sl = gbl(p.entry_slot)[e];
-            gbl(p.out)[p.dst ? gbl(p.dst)[e] : e] = tile[(sl >> 5) * kShStage + (sl & 31)];
+            sddmm_emit(p, p.dst ? (uint32_t)gbl(p.dst)[e] : (uint32_t)e, tile[(sl >> 5) * kShStage + (sl & 31)]);
         }
         wave_lds_sync();
     }
@@ -1353,11 +1390,7 @@ template <bool BF>
 __global__ __launch_bounds__(256) void sgd_kernel(uint16_t *__restrict__ w, const float *__restrict__ g, long n8,
                                                   long n, float lr)
 {
-    auto upd = [&](uint32_t bits, float gv) -> uint32_t {
-        const float x = BF ? __builtin_bit_cast(float, bits << 16) : (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
-        const float y = fmaf(-lr, gv, x);
-        return BF ? f32_to_bf16_bits(y) : f32_to_f16_bits(y);
-    };
+    auto upd = [&](uint32_t bits, float gv) -> uint32_t { return sgd_step(bits, gv, lr, BF); };
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (long)gridDim.x * blockDim.x) {
         const u32x4 wv = reinterpret_cast<const u32x4 *>(w)[i];
         const float4 g0 = reinterpret_cast<const float4 *>(g)[2 * i], g1 = reinterpret_cast<const float4 *>(g)[2 * i + 1];
@@ -1548,6 +1581,9 @@ __device__ __forceinline__ SddmmParams to_sddmm(const BatchItemDev &d)
     p.x_bytes = d.b_bytes;
     p.nnz = (uint32_t)d.nnz;
     p.dst = d.dst;
+    p.sgd_w = d.sgd_w;
+    p.sgd_lr = d.sgd_lr;
+    p.sgd_bf16 = d.sgd_bf16;
     return p;
 }
 
@@ -1969,6 +2005,9 @@ tca_status launch_sddmm(const tca_plan *p, tca_dtype g_dtype, const void *G, int
     if (st != TCA_OK) return st;
     if (n_units == 0) return TCA_OK;
     SddmmParams prm;
+    prm.sgd_w = nullptr;   // the fused weight update is a batch item option
+    prm.sgd_lr = 0.0f;
+    prm.sgd_bf16 = 0;
     prm.grp_chunk_ptr = aux->d_grp_chunk_ptr;
     prm.chunk_cols = aux->d_chunk_cols;
     prm.seg_ptr = aux->d_seg_ptr;

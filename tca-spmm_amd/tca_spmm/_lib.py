@@ -39,7 +39,8 @@ class BatchItem(ctypes.Structure):
                 ("a_dtype", ctypes.c_int), ("a", ctypes.c_void_p), ("lda", ctypes.c_int),
                 ("b_dtype", ctypes.c_int), ("b", ctypes.c_void_p), ("ldb", ctypes.c_int),
                 ("N", ctypes.c_int), ("out", ctypes.c_void_p), ("ldc", ctypes.c_int), ("a_dense", ctypes.c_int),
-                ("values_order", ctypes.c_void_p)]
+                ("values_order", ctypes.c_void_p), ("sgd_w", ctypes.c_void_p), ("sgd_w_dtype", ctypes.c_int),
+                ("sgd_lr", ctypes.c_float)]
 
 
 _vp = ctypes.c_void_p

@@ -13,7 +13,11 @@ hands the batch the weight VALUES (in p's tiled order, or CSR order when
 values_order is None); the batch owns the item's panel-dense tiles and rebuilds
 them from w at the start of every run, so an optimizer that updates w in place
 needs no repack launches of its own: run() = one prologue launch (FP16 copies
-of FP32 operands, dense tiles) + the persistent launch.
+of FP32 operands, dense tiles) + the persistent launch.  The optimizer step
+itself can ride on the SDDMM: sddmm_item(plan, G, X, dW, sgd=(w, lr)) also
+applies w <- w - lr * dW to every entry it writes (the arithmetic of
+sgd_update, bit for bit), after this run's tiles were packed from w -- so a
+training step is one run().
 """
 import ctypes
 
@@ -37,8 +41,14 @@ def spmm_item(plan, a_tiled, B, out, tile_n=0, dense=False, values_order=None):
                 values_order=values_order)
 
 
-def sddmm_item(plan, G, X, out):
-    return dict(op=OP_SDDMM, plan=plan, a=G, b=X, out=out, tile_n=0)
+def sddmm_item(plan, G, X, out, sgd=None):
+    """sgd=(w, lr): w (F16/BF16 values in the same order as out) is updated
+    in place from every dW entry the item writes: w <- round(w - lr * dW)."""
+    it = dict(op=OP_SDDMM, plan=plan, a=G, b=X, out=out, tile_n=0)
+    if sgd is not None:
+        w, lr = sgd
+        it["sgd_w"], it["sgd_lr"] = w, float(lr)
+    return it
 
 
 def _check_item(it):
@@ -58,6 +68,11 @@ def _check_item(it):
         _check_matrix(a, "G", rows=plan.n_rows, dtypes=_F16ISH + (torch.float32,))
         _check_matrix(b, "X", rows=plan.n_cols, cols=a.shape[1], dtypes=_F16ISH)
         _check_vector(out, "out", plan.nnz, (torch.float32,))
+        if it.get("sgd_w") is not None:
+            _check_vector(it["sgd_w"], "sgd weights", plan.nnz, _F16ISH)
+            _check_devices(plan, it["sgd_w"])
+    if it["op"] == OP_SPMM and it.get("sgd_w") is not None:
+        raise ValueError("sgd applies to SDDMM items")
     _check_devices(plan, a, b, out)
 
 
@@ -89,6 +104,12 @@ class Batch:
             if vo is not None:
                 self._keep.append(vo)
                 e.values_order = vo.handle
+            w = it.get("sgd_w")
+            if w is not None:
+                self._keep.append(w)
+                e.sgd_w = w.data_ptr()
+                e.sgd_w_dtype = tca_dtype(w)
+                e.sgd_lr = it["sgd_lr"]
         h = ctypes.c_void_p()
         call("tca_batch_create", len(items), arr, ctypes.byref(h))
         self._h = h

@@ -8,7 +8,9 @@ Checked on EVERY layer against the FP64 oracle (forward on the FP16 inputs,
 dX and dW on G rounded to FP16, the rounding the kernels apply), after an
 in-place weight update (the tiles follow the weights), with values in CSR
 order, across HIP-graph replays with weight updates between them, and a
-values_order of another pattern is rejected."""
+values_order of another pattern is rejected; with the optimizer step fused into
+the SDDMM (sddmm_item(sgd=...)) the weights after a run equal a separate
+tca_sgd_update on that run's dW, bit for bit."""
 import numpy as np
 import pytest
 import torch
@@ -44,13 +46,14 @@ def _setup(cuda, csr_order=False, layers=None):
     return out
 
 
-def _batch(layers, csr_order=False):
+def _batch(layers, csr_order=False, sgd_lr=None):
     items = []
     for L in layers:
         vo = None if csr_order else L["p16"]
         items.append(T.spmm_item(L["pf"], L["w"], L["X"], L["Y"], L["tile"], dense="pack", values_order=vo))
         items.append(T.spmm_item(L["pt"], L["w"], L["G"], L["dX"], L["tile_t"], dense="pack", values_order=vo))
-        items.append(T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"]))
+        sgd = None if sgd_lr is None else (L["w"], sgd_lr)
+        items.append(T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=sgd))
     return T.Batch(items)
 
 
@@ -156,6 +159,50 @@ def test_train_step_graph_replays_follow_weights(cuda):
             assert rel_l2(y.cpu().numpy(), L["Y"].cpu().numpy()) <= 1e-6, rep
             assert rel_l2(dx.cpu().numpy(), L["dX"].cpu().numpy()) <= 1e-6, rep
     _check_oracle(layers[-1])
+
+
+def test_train_step_fused_sgd(cuda):
+    # the SDDMM items also apply w <- round(w - lr * dW) (sddmm_item(sgd=...)):
+    # every run computes forward / dX / dW with the weights its prologue packed,
+    # then leaves the updated weights -- bit for bit what a separate
+    # tca_sgd_update on the run's dW gives; the next run's products follow them
+    lr = 0.0625   # large enough that the updates change most FP16 weights
+    layers = _setup(cuda, layers=(0, 5, 9, 11))
+    bt = _batch(layers, sgd_lr=lr)
+    for run in range(2):
+        before = [L["w"].clone() for L in layers]
+        bt.run()
+        torch.cuda.synchronize()
+        for L, w0 in zip(layers, before):
+            after = L["w"].clone()
+            L["w"].copy_(w0)               # the oracle check uses the weights the run multiplied with
+            _check_oracle(L)
+            want = w0.clone()
+            T.sgd_update(want, L["dW"], lr)
+            torch.cuda.synchronize()
+            assert torch.equal(after.view(torch.int16), want.view(torch.int16)), (L["case"].A.name, run)
+            assert not torch.equal(after, w0), L["case"].A.name
+            L["w"].copy_(after)
+
+
+def test_fused_sgd_bf16_weights_and_rejections(cuda):
+    L = _setup(cuda, layers=(3,))[0]
+    W = L["case"].A
+    wb = (torch.arange(W.nnz, device=cuda, dtype=torch.float32) % 97 - 48.0).div_(64.0).to(torch.bfloat16)
+    want = wb.clone()
+    T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(wb, 0.5))]).run()
+    torch.cuda.synchronize()
+    T.sgd_update(want, L["dW"], 0.5)
+    torch.cuda.synchronize()
+    assert torch.equal(wb.view(torch.int16), want.view(torch.int16))
+    with pytest.raises(ValueError):   # SpMM items take no weight update
+        T.Batch([dict(T.spmm_item(L["pf"], L["w"], L["X"], L["Y"], L["tile"], dense="pack", values_order=L["p16"]),
+                      sgd_w=L["w"], sgd_lr=0.1)])
+    with pytest.raises(ValueError):   # too short
+        T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(wb[:-1], 0.5))])
+    with pytest.raises(T.TcaError) as e:
+        T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(wb, float("nan")))])
+    assert e.value.name == "TCA_ERR_INVALID_ARG"
 
 
 def test_train_step_rejects_foreign_values_order(cuda):
