@@ -333,7 +333,9 @@ typedef struct tca_batch_item {
        out[i], sgd_w[i])) (sgd_w_dtype F16 or BF16, same order and length as
        out), the same arithmetic as tca_sgd_update.  The item's own reads (G,
        X) do not involve sgd_w, so the update may target weights whose dense
-       tiles the same batch packed in its prologue. */
+       tiles the same batch packed in its prologue (a_dense = 2); an SPMM item
+       of the batch that reads overlapping values directly (a_dense = 0) is
+       rejected with TCA_ERR_INVALID_ARG. */
     void *sgd_w;
     tca_dtype sgd_w_dtype;
     float sgd_lr;

@@ -203,6 +203,23 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             return fail(TCA_ERR_HIP, "tca_batch_create: upload failed");
     }
     items = its.data();
+    // A fused weight update must not race with a unit that reads the weights
+    // directly: SpMM items on tiled values (the packed and the caller's dense
+    // tiles are other memory) may not overlap any SDDMM item's sgd_w.
+    for (int i = 0; i < n_items; i++) {
+        const tca_batch_item &wi = items[i];
+        if (wi.op != TCA_OP_SDDMM || !wi.sgd_w || !wi.plan) continue;
+        const char *w0 = static_cast<const char *>(wi.sgd_w), *w1 = w0 + (size_t)wi.plan->host.nnz * 2;
+        for (int k = 0; k < n_items; k++) {
+            const tca_batch_item &ri = items[k];
+            if (ri.op != TCA_OP_SPMM || ri.a_dense || !ri.a || !ri.plan) continue;
+            const char *a0 = static_cast<const char *>(ri.a);
+            const char *a1 = a0 + (size_t)ri.plan->host.nnz * (ri.a_dtype == TCA_E4M3 ? 1 : 2);
+            if (a0 < w1 && w0 < a1)
+                return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SPMM item " + std::to_string(k) + " reads the weights SDDMM item " +
+                                                     std::to_string(i) + " updates (sgd_w) in the same launch");
+        }
+    }
     std::vector<tca::BatchItemDev> dev(n_items);
     std::vector<tca::BatchUnitDev> units;
     std::vector<double> cost;

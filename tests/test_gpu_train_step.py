@@ -203,6 +203,11 @@ def test_fused_sgd_bf16_weights_and_rejections(cuda):
     with pytest.raises(T.TcaError) as e:
         T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(wb, float("nan")))])
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+    # an SpMM item reading the updated weights directly (tiled values) would race
+    with pytest.raises(T.TcaError) as e:
+        T.Batch([T.spmm_item(L["p16"], L["w"], L["X"], L["Y"], 64),
+                 T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(L["w"], 0.5))])
+    assert e.value.name == "TCA_ERR_INVALID_ARG"
 
 
 def test_train_step_rejects_foreign_values_order(cuda):
