@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define TCA_VERSION 100
+#define TCA_VERSION 101
 
 typedef enum tca_status {
     TCA_OK = 0,
@@ -342,6 +342,9 @@ typedef struct tca_batch_item {
 } tca_batch_item;
 typedef struct tca_batch tca_batch;
 tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch);
+/* sizeof(tca_batch_item) as this library was built: bindings that restate the
+ * struct (ctypes) compare it at load time. */
+size_t tca_batch_item_size(void);
 tca_status tca_batch_run(tca_batch *batch, void *stream);
 /* One stage of tca_batch_run: 1 = the prologue (FP16 copies of F32 operands and
  * dense tiles of a_dense = 2 items; nothing is launched when the batch has

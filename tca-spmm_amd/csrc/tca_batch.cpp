@@ -39,6 +39,8 @@ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 
 }  // namespace
 
+extern "C" size_t tca_batch_item_size(void) { return sizeof(tca_batch_item); }
+
 extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items, tca_batch **batch)
 {
     if (!batch || n_items <= 0 || !items) return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: bad argument");

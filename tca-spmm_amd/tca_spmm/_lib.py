@@ -56,6 +56,7 @@ SIGNATURES = {
     "tca_status_string": (ctypes.c_char_p, [_i]),
     "tca_last_error": (ctypes.c_char_p, []),
     "tca_version": (_i, []),
+    "tca_batch_item_size": (ctypes.c_size_t, []),
     "tca_host_free": (None, [_vp]),
     "tca_smtx_read": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                            ctypes.POINTER(_i32p), ctypes.POINTER(_i32p)]),
@@ -128,6 +129,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.tca_batch_item_size() != ctypes.sizeof(BatchItem):
+            raise ImportError(f"{LIB_PATH}: tca_batch_item is {L.tca_batch_item_size()} bytes in the library, "
+                              f"{ctypes.sizeof(BatchItem)} in this binding (stale build? rebuild the library)")
         _lib = L
     return _lib
 

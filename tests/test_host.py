@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # and the ctypes table covers all of them
     assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
-    assert L.tca_version() == 100
+    assert L.tca_version() == 101
+    assert L.tca_batch_item_size() == ctypes.sizeof(_lib.BatchItem)
 
 
 def test_status_strings():
