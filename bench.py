@@ -235,21 +235,39 @@ class TrainStep:
 
     def parity_gate(self, O):
         """Fused outputs of every layer against the per-call tiled launches
-        (FP32 summation order; dW mapped through CSR order) and two layers
-        against the FP64 oracle."""
+        (FP32 summation order; dW mapped through CSR order) on two steps in a
+        row -- the second multiplies with the weights the first step's fused
+        update left, the per-call path applies the same update in between --
+        and two layers of the first step against the FP64 oracle."""
         torch, np = self.torch, self.np
-        self.batch.run()
-        torch.cuda.synchronize()
-        fused = [(L["Y"].clone(), L["dX"].clone(), L["f"]["plan"].unpack(L["f"]["dW"])) for L in self.layers]
-        self.products("t")
-        torch.cuda.synchronize()
+
+        def weights(path):   # every layer's weights in CSR order
+            return torch.cat([L[path]["plan"].unpack(L[path]["w"]) for L in self.layers]).view(torch.int16)
+
         worst = 0.0
-        for L, (y, dx, dw) in zip(self.layers, fused):
-            for got, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["t"]["plan"].unpack(L["t"]["dW"]))):
-                r = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
-                worst = max(worst, r)
-                if r > 1e-6:
-                    raise RuntimeError(f"{L['case'].A.name}: fused step differs from the per-call launches ({r:.2e})")
+        for step in range(2):
+            w_before = weights("f")
+            if step == 1:    # both paths multiply with the same updated weights
+                if not torch.equal(weights("t"), w_before):
+                    raise RuntimeError("per-call weights differ from the fused step's after one update")
+                log(f"parity gate: {int((w_before != w_first).sum())} of {w_before.numel()} weights changed by "
+                    "the first step's update")
+            w_first = w_before
+            self.batch.run()
+            torch.cuda.synchronize()
+            got = [(L["Y"].clone(), L["dX"].clone(), L["f"]["plan"].unpack(L["f"]["dW"])) for L in self.layers]
+            if step == 0:
+                fused = got
+            self.products("t")
+            torch.cuda.synchronize()
+            for L, (y, dx, dw) in zip(self.layers, got):
+                for g, ref in ((y, L["Y"]), (dx, L["dX"]), (dw, L["t"]["plan"].unpack(L["t"]["dW"]))):
+                    r = float(torch.linalg.norm(g - ref) / max(float(torch.linalg.norm(ref)), 1e-30))
+                    worst = max(worst, r)
+                    if r > 1e-6:
+                        raise RuntimeError(f"{L['case'].A.name}: fused step {step} differs from the per-call "
+                                           f"launches ({r:.2e})")
+            self.sgd("t")
         oracle_err = {}
         for i in (0, len(self.layers) - 1):
             L = self.layers[i]
