@@ -162,12 +162,16 @@ def record(A, N, results, op="spmm", path=None, layout="tiled"):
         json.dump(tab, f, indent=1)
 
 
-def tune_batch(cases, candidates=BATCH_DENSE_CANDIDATES, iters=20, passes=1, device="cuda", log=None):
+def tune_batch(cases, candidates=BATCH_DENSE_CANDIDATES, iters=20, passes=1, device="cuda", log=None,
+               start="table", sgd_lr=1e-6):
     """Per-layer choice of the fused training step's batched SpMM variants:
     forward (op "spmm") and dX (op "spmm_t") of every layer, by coordinate
     descent on the measured time of the whole batch run (prologue packing the
     dense tiles from the weight values + the persistent launch over forward,
-    dX and dW of all layers).  cases: workloads.SpmmCase with extra["G"].
+    dX and dW of all layers, the SDDMM items applying the SGD update with
+    sgd_lr; None: no update).  cases: workloads.SpmmCase with extra["G"].
+    start: "table" = the choices choice_for(batch=True, dense=True) gives now
+    (recorded ones or the rule), "rule" = 32 rows x 64 columns everywhere.
     Returns (choices, us): choices[i] = {"fwd": (rpw, tile), "dx": (rpw, tile)}
     and the step's batch time with them; every candidate tried is timed."""
     import numpy as np
@@ -208,7 +212,7 @@ def tune_batch(cases, candidates=BATCH_DENSE_CANDIDATES, iters=20, passes=1, dev
             keep += [w, dW]
             items += [spmm_item(pw, w, L["X"], L["Y"], ch["fwd"][1], dense="pack", values_order=pw),
                       spmm_item(pt, w, L["G"], L["dX"], ch["dx"][1], dense="pack", values_order=pw),
-                      sddmm_item(pw, L["G"], L["X"], dW)]
+                      sddmm_item(pw, L["G"], L["X"], dW, sgd=None if sgd_lr is None else (w, sgd_lr))]
         b = Batch(items)
         for _ in range(3):
             b.run()
@@ -227,8 +231,12 @@ def tune_batch(cases, candidates=BATCH_DENSE_CANDIDATES, iters=20, passes=1, dev
     choice = []
     for L in layers:
         W, N = L["case"].A, L["case"].N
-        t = 64 if N >= 64 else 32
-        choice.append({"fwd": (32, t), "dx": (32, t)})
+        if start == "table":
+            choice.append({"fwd": tuple(choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)),
+                           "dx": tuple(choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True))})
+        else:
+            t = 64 if N >= 64 else 32
+            choice.append({"fwd": (32, t), "dx": (32, t)})
     best = measure(choice)
     if log:
         log(f"batch start {best:.1f} us")
@@ -286,6 +294,7 @@ def main(argv=None):
     ap.add_argument("--layouts", default="tiled,dense")
     ap.add_argument("--batch", action="store_true", help="also tune the batched training step of batch12")
     ap.add_argument("--batch-only", action="store_true", help="only tune the batched training step (keeps the table)")
+    ap.add_argument("--passes", type=int, default=1, help="coordinate-descent passes of the batch tuning")
     args = ap.parse_args(argv)
     global _table
     dev = torch.device("cuda:0")
@@ -293,7 +302,7 @@ def main(argv=None):
         _table = None
         _load()
         cases = workloads.batch12()
-        ch, us = tune_batch(cases, log=lambda m: print(m, flush=True))
+        ch, us = tune_batch(cases, passes=args.passes, log=lambda m: print(m, flush=True))
         record_batch(cases, ch, us, path=args.out)
         return
     _table = {"entries": []}
@@ -327,7 +336,7 @@ def main(argv=None):
                       f"tile={rt[0]['tile_n']} {rt[0]['us']:.1f} us (worst {rt[-1]['us']:.1f})", flush=True)
     if args.batch:
         cases = workloads.batch12()
-        ch, us = tune_batch(cases, log=lambda m: print(m, flush=True))
+        ch, us = tune_batch(cases, passes=args.passes, log=lambda m: print(m, flush=True))
         record_batch(cases, ch, us, path=args.out)
 
 
