@@ -618,6 +618,135 @@ def bench_precision(torch, np, T, O, dev, args):
     return out
 
 
+def bench_linear(torch, np, T, O, dev, args):
+    """One sparse linear layer at full size: W 4096x1024 at 0.9 (lognormal
+    rows), X 1024x2048 FP16, upstream gradient G 4096x2048 FP32.  Forward, dX
+    and dW per call (tuned shapes, panel-dense tiles; G passed as FP32, which
+    the kernels round to FP16 on load),
+    the same three as one batch run with the fused SGD update (a one-layer
+    training step: prologue + persistent launch), and SparseLinear's forward +
+    backward through torch.autograd; each checked against the FP64 oracle
+    (rel-L2 <= 1e-5), each time the mean of back-to-back launches behind a
+    queue backlog; hipBLASLt's dense FP16 GEMM of the densified W beside the
+    forward."""
+    from tca_spmm import workloads
+    from tca_spmm.workloads import alg_bytes_dw, alg_bytes_dx, alg_bytes_fwd
+    case = workloads.sparse_linear(seed=0)
+    W, N = case.A, case.B.shape[1]
+    M, K, nnz = W.n_rows, W.n_cols, W.nnz
+    G32 = case.extra["G"]
+    a64 = W.values.view(np.float16).astype(np.float64)
+    x64 = case.B.view(np.float16).astype(np.float64)
+    g64 = G32.astype(np.float64)
+    g64q = G32.astype(np.float16).astype(np.float64)   # the batch reads G rounded to FP16
+    rpT, cT, perm = O.transpose_csr(M, K, W.row_offset, W.col_idx)
+    ref_y = O.spmm_f64(M, N, W.row_offset, W.col_idx, a64, x64)
+    ref = {"fwd": ref_y, "dx": O.spmm_f64(K, N, rpT, cT, a64[perm], g64),
+           "dw": O.sddmm_f64(M, N, W.row_offset, W.col_idx, g64, x64)}
+    ref_q = {"fwd": ref_y, "dx": O.spmm_f64(K, N, rpT, cT, a64[perm], g64q),
+             "dw": O.sddmm_f64(M, N, W.row_offset, W.col_idx, g64q, x64)}
+    X = torch.from_numpy(case.B.view(np.int16)).view(torch.float16).to(dev)
+    G = torch.from_numpy(G32).to(dev)
+    v_csr = torch.from_numpy(W.values.view(np.int16)).view(torch.float16).to(dev)
+    Y = torch.empty((M, N), dtype=torch.float32, device=dev)
+    dX = torch.empty((K, N), dtype=torch.float32, device=dev)
+    p16 = T.Plan(M, K, W.row_offset, W.col_idx, 16)
+    w16 = p16.pack(v_csr)
+    dW = torch.empty(nnz, dtype=torch.float32, device=dev)
+    alg = {"fwd": alg_bytes_fwd(M, K, nnz, N), "dx": alg_bytes_dx(M, K, nnz, N), "dw": alg_bytes_dw(M, K, nnz, N)}
+    flops = 2 * nnz * N
+    out = {"config": {"M": M, "K": K, "nnz": nnz, "N": N, "alg_bytes": alg}}
+
+    def check(tag, got, key, refs):
+        r = rel_l2(np, got, refs[key])
+        if r > 1e-5:
+            raise RuntimeError(f"sparse linear {tag} {key}: rel-L2 {r:.2e} vs the FP64 oracle")
+        return r
+
+    def entry(ms, key, err):
+        return {"us": ms * 1e3, "tflops": flops / (ms * 1e-3) / 1e12,
+                "hbm_frac": alg[key] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rel_l2_vs_fp64": err}
+
+    # per call
+    r_f, t_f = T.autotune.choice_for(M, K, nnz, N, dense=True)
+    r_t, t_t = T.autotune.choice_for(K, M, nnz, N, op="spmm_t", dense=True)
+    pf = p16 if r_f == 16 else T.Plan(M, K, W.row_offset, W.col_idx, r_f)
+    pt = p16.transpose(r_t)
+    df = pf.pack_dense(pf.pack(v_csr))
+    dt = pt.pack_dense(w16)
+    calls = {"fwd": lambda: T.spmm(pf, df, X, out=Y, tile_n=t_f, dense=True),
+             "dx": lambda: T.spmm(pt, dt, G, out=dX, tile_n=t_t, dense=True),
+             "dw": lambda: T.sddmm(p16, G, X, out=dW)}
+    per_call = {}
+    for key, fn in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        got = {"fwd": Y, "dx": dX, "dw": None}[key]
+        got = p16.unpack(dW).cpu().numpy() if key == "dw" else got.cpu().numpy()
+        e = check("per call", got, key, ref_q)   # the kernels round an FP32 G to FP16 on load
+        per_call[key] = entry(time_eager(torch, fn, 20, backlog=True), key, e)
+    per_call["shapes"] = {"fwd": [r_f, t_f], "dx": [r_t, t_t]}
+    out["per_call"] = per_call
+    # one batch run: the layer's training step (dense tiles packed from the
+    # weights, fused SGD update with a zero rate so that the weights stay put)
+    rb_f, tb_f = T.autotune.choice_for(M, K, nnz, N, batch=True, dense=True)
+    rb_t, tb_t = T.autotune.choice_for(K, M, nnz, N, op="spmm_t", batch=True, dense=True)
+    bf = p16 if rb_f == 16 else T.Plan(M, K, W.row_offset, W.col_idx, rb_f)
+    bt = p16.transpose(rb_t)
+    wb = bf.pack(v_csr)
+    dWb = torch.empty(nnz, dtype=torch.float32, device=dev)
+    batch = T.Batch([T.spmm_item(bf, wb, X, Y, tb_f, dense="pack", values_order=bf),
+                     T.spmm_item(bt, wb, G, dX, tb_t, dense="pack", values_order=bf),
+                     T.sddmm_item(bf, G, X, dWb, sgd=(wb, 0.0))])
+    batch.run()
+    torch.cuda.synchronize()
+    errs = {"fwd": check("batch", Y.cpu().numpy(), "fwd", ref_q), "dx": check("batch", dX.cpu().numpy(), "dx", ref_q),
+            "dw": check("batch", bf.unpack(dWb).cpu().numpy(), "dw", ref_q)}
+    ms_b = time_eager(torch, batch.run, 20, backlog=True)
+    out["batch_step"] = {"us": ms_b * 1e3, "tflops": 3 * flops / (ms_b * 1e-3) / 1e12,
+                         "hbm_frac": sum(alg.values()) / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "main_launch_us": time_eager(torch, lambda: batch.run(stage=2), 20, backlog=True) * 1e3,
+                         "prologue_us": time_eager(torch, lambda: batch.run(stage=1), 20, backlog=True) * 1e3,
+                         "rel_l2_vs_fp64_on_fp16_g": errs, "shapes": {"fwd": [rb_f, tb_f], "dx": [rb_t, tb_t]},
+                         "per_call_sum_us": sum(per_call[k]["us"] for k in ("fwd", "dx", "dw"))}
+    batch.close()
+    # SparseLinear through torch.autograd (forward, then backward with G)
+    lin = T.SparseLinear(M, K, W.row_offset, W.col_idx, W.values, rows_per_wave=r_f, rows_per_wave_t=r_t,
+                         tile_n=t_f, device=dev, dense=True)
+    Xg = X.clone().requires_grad_(True)
+
+    res = {}
+
+    def fwd_bwd():
+        lin.weight.grad = None
+        Xg.grad = None
+        y = lin(Xg)
+        y.backward(G)
+        res["y"] = y
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    # the gradients come back in the FP16 of the weights and of X (G rounded to
+    # FP16 on the way in): their rounding alone is ~3e-4 rel-L2
+    e_y = check("autograd", res["y"].detach().cpu().numpy(), "fwd", ref)
+    e_g = {k: rel_l2(np, v, ref_q[k]) for k, v in
+           (("dx", Xg.grad.float().cpu().numpy()), ("dw", lin.plan.unpack(lin.weight.grad).float().cpu().numpy()))}
+    if max(e_g.values()) > 1e-3:
+        raise RuntimeError(f"sparse linear autograd gradients vs the FP64 oracle: {e_g}")
+    out["autograd_fwd_bwd"] = {"us": time_eager(torch, fwd_bwd, 10) * 1e3, "rel_l2_fwd_vs_fp64": e_y,
+                               "rel_l2_fp16_grads_vs_fp64": e_g, "grad_dtype": str(lin.weight.grad.dtype)}
+    if not args.no_libs:
+        dense = torch.zeros((M, K), dtype=torch.float16, device=dev)
+        rows = torch.repeat_interleave(torch.arange(M, device=dev), torch.from_numpy(np.diff(W.row_offset)).to(dev))
+        dense[rows, torch.from_numpy(W.col_idx).to(dev).long()] = v_csr
+        try:
+            out["hipblaslt_dense_fwd_us"] = T.hipblaslt_gemm(M, N, K, dense, X, Y, n_iter=20) * 1e3
+        except Exception as ex:
+            log("hipBLASLt failed:", ex)
+        del dense
+    return out
+
+
 def bench_plain_cpu(torch, np, T, O, dev, cores):
     """512x512 at 0.9 (Binomial rows), N=256: the reference's CPU-runnable
     case, on the GPU (tuned per-call launch) and on the CPU oracle (all
@@ -650,8 +779,8 @@ def bench_plain_cpu(torch, np, T, O, dev, cores):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="suite,precision,plain_cpu",
-                    help="comma list of extra sections: suite, precision, plain_cpu")
+    ap.add_argument("--sections", default="suite,precision,linear,plain_cpu",
+                    help="comma list of extra sections: suite, precision, linear, plain_cpu")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
@@ -712,6 +841,8 @@ def main():
         extra["plain_gpu_suite"] = bench_plain_gpu_suite(torch, np, T, O, dev, args)
     if "precision" in args.sections:
         extra["precision_4096"] = bench_precision(torch, np, T, O, dev, args)
+    if "linear" in args.sections:
+        extra["sparse_linear_4096x1024"] = bench_linear(torch, np, T, O, dev, args)
     if "plain_cpu" in args.sections:
         extra["plain_cpu_512"] = bench_plain_cpu(torch, np, T, O, dev, cores)
 
