@@ -334,8 +334,9 @@ typedef struct tca_batch_item {
        out), the same arithmetic as tca_sgd_update.  The item's own reads (G,
        X) do not involve sgd_w, so the update may target weights whose dense
        tiles the same batch packed in its prologue (a_dense = 2); an SPMM item
-       of the batch that reads overlapping values directly (a_dense = 0) is
-       rejected with TCA_ERR_INVALID_ARG. */
+       of the batch that reads overlapping values directly (a_dense = 0), or a
+       second SDDMM item updating overlapping weights, is rejected with
+       TCA_ERR_INVALID_ARG. */
     void *sgd_w;
     tca_dtype sgd_w_dtype;
     float sgd_lr;

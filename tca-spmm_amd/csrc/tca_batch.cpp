@@ -207,11 +207,20 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     items = its.data();
     // A fused weight update must not race with a unit that reads the weights
     // directly: SpMM items on tiled values (the packed and the caller's dense
-    // tiles are other memory) may not overlap any SDDMM item's sgd_w.
+    // tiles are other memory) may not overlap any SDDMM item's sgd_w, and no
+    // two SDDMM items may update overlapping weights.
     for (int i = 0; i < n_items; i++) {
         const tca_batch_item &wi = items[i];
         if (wi.op != TCA_OP_SDDMM || !wi.sgd_w || !wi.plan) continue;
         const char *w0 = static_cast<const char *>(wi.sgd_w), *w1 = w0 + (size_t)wi.plan->host.nnz * 2;
+        for (int k = i + 1; k < n_items; k++) {
+            const tca_batch_item &oi = items[k];
+            if (oi.op != TCA_OP_SDDMM || !oi.sgd_w || !oi.plan) continue;
+            const char *o0 = static_cast<const char *>(oi.sgd_w), *o1 = o0 + (size_t)oi.plan->host.nnz * 2;
+            if (o0 < w1 && w0 < o1)
+                return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: SDDMM items " + std::to_string(i) + " and " +
+                                                     std::to_string(k) + " update overlapping weights (sgd_w)");
+        }
         for (int k = 0; k < n_items; k++) {
             const tca_batch_item &ri = items[k];
             if (ri.op != TCA_OP_SPMM || ri.a_dense || !ri.a || !ri.plan) continue;

@@ -208,6 +208,13 @@ def test_fused_sgd_bf16_weights_and_rejections(cuda):
         T.Batch([T.spmm_item(L["p16"], L["w"], L["X"], L["Y"], 64),
                  T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(L["w"], 0.5))])
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+    # so would two updates of overlapping weights
+    big = torch.zeros(W.nnz + 64, dtype=torch.float16, device=cuda)
+    dW2 = torch.empty_like(L["dW"])
+    with pytest.raises(T.TcaError) as e:
+        T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(big[:W.nnz], 0.5)),
+                 T.sddmm_item(L["p16"], L["G"], L["X"], dW2, sgd=(big[8:W.nnz + 8], 0.5))])
+    assert e.value.name == "TCA_ERR_INVALID_ARG"
 
 
 def test_train_step_rejects_foreign_values_order(cuda):
