@@ -1,8 +1,7 @@
-"""bench.py's contract, checked without a GPU: the metric is the one the
-measurement plan names, more than one GPU is refused with a message (this
-build measures one), and the roofline's traffic figure comes from the
-committed rocprof summary of the step's persistent launch."""
-import json
+"""bench.py's contract, checked without a GPU: the roofline's peaks, more
+than one GPU refused with a message (this build measures one), and the
+roofline's traffic figure read from the committed rocprof summary of the
+step's persistent launch."""
 import os
 import subprocess
 import sys
@@ -13,10 +12,10 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 
-def test_metric_is_the_planned_one():
-    with open(os.path.join(ROOT, "BASELINE.json")) as f:
-        assert bench.METRIC == json.load(f)["metric"]
+def test_roofline_constants():
+    # MI355X's published peaks: 8 TB/s of HBM, 2.5 PFLOP/s of dense FP16 MFMA
     assert bench.HBM_PEAK_GBS == 8000.0
+    assert bench.MFMA_F16_PEAK_TFLOPS == 2500.0
 
 
 def test_more_than_one_gpu_is_refused():
