@@ -571,14 +571,12 @@ def bench_precision(torch, np, T, O, dev, args):
     rpd = torch.from_numpy(rp).to(dev)
     qa, sa = T.quantize_rows_e4m3(rpd, va)
     qb, sb = T.quantize_cols_e4m3(vb)
-    # E4M3 runs on the one-wave-per-group kernels (16 / 32 / 64 rows): plans
-    # of 128 / 256 rows take 16-bit operands only
+    # E4M3 tiled values run on the one-wave-per-group kernels (16 / 32 / 64
+    # rows; the wide kernels of 128 / 256-row plans take 16-bit values); the
+    # dense tiles run on the block kernels like FP16
     if rpw >= 128:
         rpw, tile = 32, 64
         plan = T.Plan(A.n_rows, A.n_cols, rp, ci, rpw)
-    if rpw_d >= 128:
-        rpw_d, tile_d = 32, 64
-        plan_d = T.Plan(A.n_rows, A.n_cols, rp, ci, rpw_d)
     ta = plan.pack(qa)
     da = plan_d.pack_dense(plan_d.pack(qa))
     ms_d = time_eager(torch, lambda: T.spmm(plan_d, da, qb, out=C, row_scale=sa, col_scale=sb, tile_n=tile_d, dense=True), 50, backlog=True)

@@ -193,16 +193,20 @@ tca_status tca_pack_dense(const tca_plan *plan, tca_dtype dtype, const void *d_t
 /*   row_scale[M] / col_scale[N] (may be NULL) multiply the FP32 result:     */
 /*   C[r,n] = row_scale[r] * col_scale[n] * sum_k A[r,k] B[k,n] (E4M3).      */
 /*   tile_n in {16, 32, 64, 128}, or 0 to pick from the built-in table.      */
-/*   Plans of 128 / 256 rows per group: tile_n 64 or 128, F16/BF16 A only.   */
+/*   Plans of 128 / 256 rows per group: tile_n 64 or 128, F16/BF16 A only   */
+/*   (tca_spmm_dense also takes E4M3 there).                                 */
 /* ------------------------------------------------------------------------ */
 tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_tiled,
                     tca_dtype b_dtype, const void *d_B, int ldb, int N,
                     float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,
                     int tile_n, void *stream);
 /* tca_spmm with A as panel-dense tiles (tca_pack_dense), forward or
- * transposed plan (128 / 256 rows per group: F16 / BF16 A); same arguments
- * and results (bit for bit: the MFMAs see the same operands in the same
- * order). */
+ * transposed plan; same arguments.  Plans of 16 / 32 / 64 rows give the
+ * results of tca_spmm bit for bit (the MFMAs see the same operands in the same
+ * order); plans of 128 / 256 rows (F16 / BF16 A with an F16 / BF16 / F32 B, or
+ * E4M3 A with an E4M3 B, tile_n 64 or 128) run the block kernels, whose two K
+ * groups add their partial sums in a fixed order: the same results to FP32
+ * summation order, the same bits from launch to launch. */
 tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_dense,
                           tca_dtype b_dtype, const void *d_B, int ldb, int N,
                           float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,

@@ -788,33 +788,50 @@ __global__ __launch_bounds__(64 * WM) void spmm_wide_kernel(SpmmParams p)
 // Per chunk a K group reads 64*NT bytes of B and PPG KiB of A for PPG*NT/16
 // MFMAs, against 64*NT + 1 KiB per panel for one wave per 16..64 rows.
 // ---------------------------------------------------------------------------
-template <int NT, int WM>
+template <int NT, int WM, int ES = 2>
 struct BlockLds {
-    static constexpr int LDB = NT + 16;                  // B row pitch (elements)
-    static constexpr int BTILE = 32 * LDB * 2;           // one chunk's B tile (bytes, rows by tr_row)
-    static constexpr int SEGS = NT / 8;                  // 16-byte pieces per B row
+    // B row pitch (bytes): 16-bit rows NT+16 elements (rows by tr_row); 8-bit
+    // rows NT+16 bytes, so that the eight 16-byte row windows a transposing
+    // 8-bit read takes per lane group, and the next group's eight, fall on
+    // distinct banks
+    static constexpr int PITCH = (NT + 16) * ES;
+    static constexpr int BTILE = 32 * PITCH;             // one chunk's B tile (bytes)
+    static constexpr int SEGS = NT / 8;                  // 8-element pieces per B row
     static constexpr int PPT = 32 * SEGS / (WM * 64);    // pieces per thread
     static_assert(PPT >= 1 && (32 * SEGS) % (WM * 64) == 0, "B pieces must divide over the waves");
 };
-template <int PPW, int NT, int WM, int KS>
+template <int PPW, int NT, int WM, int KS, int ES = 2>
 constexpr int block_lds_bytes()
 {
-    return cmax(KS * 2 * BlockLds<NT, WM>::BTILE, (KS - 1) * WM * (NT / 16) * 1024);
+    return cmax(KS * 2 * BlockLds<NT, WM, ES>::BTILE, (KS - 1) * WM * (NT / 16) * 1024);
+}
+
+// Transposing 8-bit LDS read (ds_read_b64_tr_b8, gfx950): in each 16-lane
+// group, lane 2r+j addresses row r, bytes 8j..8j+7 of an 8-row x 16-byte
+// block, and lane i receives byte i of the eight rows -- 8 consecutive k of
+// one n, an FP8 MFMA B operand (lane layout checked on the MI355X).
+typedef int i32x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 lds_read_tr8(const uint8_t *lds, int byte_off)
+{
+    const i32x2v v = __builtin_amdgcn_ds_read_tr8_b64_v2i32((__attribute__((address_space(3))) i32x2v *)(lds + byte_off));
+    return __builtin_bit_cast(uint2, v);
 }
 
 template <int PPW, int NT, int WM, int KS, int D, tca_dtype AT, tca_dtype BT>
 __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int tile, int u_lo, int u_hi, bool atomic,
                                                 uint8_t *smem_all)
 {
-    using L = BlockLds<NT, WM>;
+    constexpr int ES = Elt<AT>::ES;                     // 2: F16 / BF16, 1: E4M3 (A and B)
+    using L = BlockLds<NT, WM, ES>;
     using RL = RowLoad<AT, BT>;
     using RB = RowBuf<AT, BT>;
-    using Oct = uint4;
+    using Oct = typename Elt<AT>::Oct;
     constexpr int NSUB = NT / 16;
     constexpr int PPT = L::PPT;
     constexpr int PPG = WM * PPW;                       // panels of the plan's group
     constexpr int RED = WM * NSUB * 1024;               // one panel of a K group's waves (bytes)
-    static_assert(Elt<AT>::ES == 2, "block units take 16-bit A");
+    constexpr int ATILE = 512 * ES;                     // one 16 x 32 A tile (bytes)
+    static_assert(ES == 2 || BT == AT, "8-bit block units take an 8-bit B");
     static_assert(D % 2 == 0, "the LDS buffer of a step is its register stage's parity");
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int h = wv / WM, w = wv - h * WM;
@@ -826,7 +843,7 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
     const int n0 = tile * NT;
     const Rsrc r_b = make_rsrc(p.B, p.b_bytes);
     const Rsrc r_cols = make_rsrc(p.chunk_cols, (uint32_t)p.n_chunks * 128u);
-    const Rsrc r_a = make_rsrc(p.a, (uint32_t)p.n_chunks * (uint32_t)(PPG * 1024));
+    const Rsrc r_a = make_rsrc(p.a, (uint32_t)p.n_chunks * (uint32_t)(PPG * ATILE));
     const uint32_t ldb_bytes = (uint32_t)p.ldb * RB::BES;
     // this thread's B pieces: row pk, 8-column segment, and where they go in LDS
     int pk[PPT];
@@ -835,16 +852,19 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const int pc = tid + j * WM * 64;
-        pk[j] = piece_row<L::SEGS>(pc / L::SEGS);
+        pk[j] = ES == 2 ? piece_row<L::SEGS>(pc / L::SEGS) : pc / L::SEGS;
         const int seg = pc % L::SEGS;
         const int ncol = n0 + 8 * seg;
         pok[j] = ncol < p.N;
         pcol[j] = (uint32_t)ncol * RB::BES;
-        plds[j] = (uint32_t)(tr_row(pk[j], L::LDB * 2) + 16 * seg);
+        plds[j] = (uint32_t)(ES == 2 ? tr_row(pk[j], L::PITCH) + 16 * seg : pk[j] * L::PITCH + 8 * seg);
     }
+    // MFMA B operand reads: 16-bit, two 4-row transposing reads per 16 columns;
+    // 8-bit, one 8-row read per 16 columns (lane group q: rows 8q..8q+7)
     const int brow = 8 * (lane >> 4) + ((lane & 15) >> 2);
-    const int boff_lo = tr_row(brow, L::LDB * 2) + 8 * (lane & 3), boff_hi = tr_row(brow + 4, L::LDB * 2) + 8 * (lane & 3);
-    const uint32_t a_lane = (uint32_t)(w * PPW) * 1024u + (uint32_t)lane * 16u;
+    const int boff_lo = tr_row(brow, L::PITCH) + 8 * (lane & 3), boff_hi = tr_row(brow + 4, L::PITCH) + 8 * (lane & 3);
+    const int boff8 = (8 * (lane >> 4) + ((lane & 15) >> 1)) * L::PITCH + 8 * (lane & 1);
+    const uint32_t a_lane = (uint32_t)(w * PPW) * (uint32_t)ATILE + (uint32_t)lane * (uint32_t)(8 * ES);
 
     f32x4 acc[PPW][NSUB];
 #pragma unroll
@@ -880,8 +900,11 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
     };
     auto load_a = [&](int c, Oct (&aop)[PPW]) {
 #pragma unroll
-        for (int i = 0; i < PPW; i++)
-            aop[i] = buf_b128(r_a, (uint32_t)c * (uint32_t)(PPG * 1024) + a_lane + (uint32_t)i * 1024u);
+        for (int i = 0; i < PPW; i++) {
+            const uint32_t off = (uint32_t)c * (uint32_t)(PPG * ATILE) + a_lane + (uint32_t)(i * ATILE);
+            if constexpr (ES == 2) aop[i] = buf_b128(r_a, off);
+            else aop[i] = buf_b64(r_a, off);
+        }
     };
     auto put_b = [&](uint8_t *buf, const typename RB::Raw (&rows)[PPT]) {
 #pragma unroll
@@ -890,9 +913,13 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
     auto read_frag = [&](const uint8_t *buf, BFrag &f) {
 #pragma unroll
         for (int sb = 0; sb < NSUB; sb++) {
-            const uint2 lo = lds_read_tr16(buf, boff_lo + 32 * sb);
-            const uint2 hi = lds_read_tr16(buf, boff_hi + 32 * sb);
-            f.b[sb] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            if constexpr (ES == 2) {
+                const uint2 lo = lds_read_tr16(buf, boff_lo + 32 * sb);
+                const uint2 hi = lds_read_tr16(buf, boff_hi + 32 * sb);
+                f.b[sb] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                f.b[sb] = lds_read_tr8(buf, boff8 + 16 * sb);
+            }
         }
     };
     // Every K group runs `iters` steps (the shorter ones idle in their last);
@@ -984,7 +1011,7 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
 template <int PPW, int NT, int WM, int KS, tca_dtype AT, tca_dtype BT>
 __global__ __launch_bounds__(64 * WM * KS) void spmm_block_kernel(SpmmParams p)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t smem[block_lds_bytes<PPW, NT, WM, KS>()];
+    __shared__ __attribute__((aligned(16))) uint8_t smem[block_lds_bytes<PPW, NT, WM, KS, Elt<AT>::ES>()];
     int ui, tile;
     if (!xcd_work(p, blockIdx.x, ui, tile)) return;
     const int4 u = p.units[ui];   // (group, pieces, c_lo, c_hi)
@@ -1068,12 +1095,13 @@ SpmmFn pick_block(int rpw, int nt)
     return nullptr;
 }
 
-// Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A on
-// the block kernels).
+// Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A, or
+// E4M3 A and B, on the block kernels).
 SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt)
 {
     if (rpw >= 128) {
         if (at == TCA_F16 && bt == TCA_F16) return pick_block<TCA_F16, TCA_F16>(rpw, nt);
+        if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_block<TCA_E4M3, TCA_E4M3>(rpw, nt);
         if (at == TCA_BF16 && bt == TCA_BF16) return pick_block<TCA_BF16, TCA_BF16>(rpw, nt);
         if (at == TCA_F16 && bt == TCA_F32) return pick_block<TCA_F16, TCA_F32>(rpw, nt);
         if (at == TCA_BF16 && bt == TCA_F32) return pick_block<TCA_BF16, TCA_F32>(rpw, nt);

@@ -5,7 +5,10 @@ tile and two K groups split the chunk range.  Checked against the FP64 oracle
 tiled-values launch of the same plan (same products, another summation order:
 rel-L2 <= 1e-6): forward and transposed plans, F16 / BF16 / an F32 B rounded on
 load, tile widths 64 / 128, ragged N and row counts, empty and one-row groups,
-groups of one chunk (one K group idle), and the 4096 x 4096 x 512 workload."""
+groups of one chunk (one K group idle), and the 4096 x 4096 x 512 workload;
+E4M3 A and B with per-row / per-column scales (8-bit tiles, B operands by
+transposing 8-bit LDS reads) against the oracle on the quantised inputs and
+scales."""
 import numpy as np
 import pytest
 import torch
@@ -106,6 +109,57 @@ def test_block_4096_workload(cuda):
             assert rel_l2(got, ref) <= 1e-5, (rpw, t)
         # two launches give the same bits (K groups add in a fixed order)
         assert torch.equal(T.spmm(plan, d, B, tile_n=64, dense=True), T.spmm(plan, d, B, tile_n=64, dense=True))
+
+
+def _e4m3_ref(A, q, s, qb, sb, N):
+    qa = O.e4m3_to_f32(q.cpu().numpy()).astype(np.float64)
+    qbf = O.e4m3_to_f32(qb.cpu().numpy()).astype(np.float64)
+    ref = O.spmm_f64(A.n_rows, N, A.row_offset, A.col_idx, qa, qbf)
+    return ref * (s.cpu().numpy().astype(np.float64)[:, None] * sb.cpu().numpy().astype(np.float64)[None, :])
+
+
+@pytest.mark.parametrize("rpw", WIDE)
+def test_block_e4m3_against_oracle(cuda, rpw):
+    for name, rp, ci, M, K in _cases():
+        if ci.size == 0:
+            continue
+        vals = gen.f32_to_f16_bits(gen.normal_f32(11, rpw, (ci.size,)))
+        A = SparseMatrix(name, M, K, rp, ci, vals)
+        q, s = T.quantize_rows_e4m3(dev(rp), dev_f16(vals))
+        plan = T.Plan(M, K, rp, ci, rpw)
+        dense = plan.pack_dense(plan.pack(q))
+        for N in (64, 136, 512):
+            qb, sb = T.quantize_cols_e4m3(dev_f16(gen.f32_to_f16_bits(gen.normal_f32(12, N, (K, N)))))
+            ref = _e4m3_ref(A, q, s, qb, sb, N)
+            for t in TILES:
+                got = T.spmm(plan, dense, qb, row_scale=s, col_scale=sb, tile_n=t, dense=True).cpu().numpy()
+                assert rel_l2(got, ref) <= 1e-5, (name, N, t)
+
+
+def test_block_e4m3_4096_workload_and_transposed(cuda):
+    case = workloads.precision()
+    A = case.A
+    q, s = T.quantize_rows_e4m3(dev(A.row_offset), dev_f16(A.values))
+    qb, sb = T.quantize_cols_e4m3(dev_f16(case.B))
+    ref = _e4m3_ref(A, q, s, qb, sb, case.N)
+    for rpw in WIDE:
+        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rpw)
+        d = plan.pack_dense(plan.pack(q))
+        for t in TILES:
+            got = T.spmm(plan, d, qb, row_scale=s, col_scale=sb, tile_n=t, dense=True).cpu().numpy()
+            assert rel_l2(got, ref) <= 1e-5, (rpw, t)
+    # transposed plan: W^T G with E4M3 values (no scales), against the oracle
+    W = workloads.batch12(seed=0)[10].A                 # 4096 x 1024 at 0.12
+    fwd = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+    qw, _ = T.quantize_rows_e4m3(dev(W.row_offset), dev_f16(W.values))
+    gq, _ = T.quantize_cols_e4m3(dev_f16(gen.f32_to_f16_bits(gen.normal_f32(13, 128, (W.n_rows, 128)))))
+    rpT, cT, perm = O.transpose_csr(W.n_rows, W.n_cols, W.row_offset, W.col_idx)
+    ref_t = O.spmm_f64(W.n_cols, 128, rpT, cT, O.e4m3_to_f32(qw.cpu().numpy()).astype(np.float64)[perm],
+                       O.e4m3_to_f32(gq.cpu().numpy()).astype(np.float64))
+    for rpw in WIDE:
+        tr = fwd.transpose(rpw)
+        got = T.spmm(tr, tr.pack_dense(fwd.pack(qw)), gq, tile_n=64, dense=True).cpu().numpy()
+        assert rel_l2(got, ref_t) <= 1e-5, rpw
 
 
 @pytest.mark.parametrize("rpw", (64, 128))

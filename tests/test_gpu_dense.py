@@ -164,8 +164,11 @@ def test_dense_rejections(cuda):
     wide = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, 128)
     q, _ = T.quantize_rows_e4m3(dev(A.row_offset), dev_f16(A.values))
     qb, _ = T.quantize_cols_e4m3(dev_f16(case.B))
-    with pytest.raises(T.TcaError) as e:                             # wide groups: F16 / BF16 only
-        T.spmm(wide, wide.pack_dense(wide.pack(q)), qb, dense=True)
+    with pytest.raises(T.TcaError) as e:                             # wide groups, tiled values: 16-bit only
+        T.spmm(wide, wide.pack(q), qb)
+    assert e.value.name == "TCA_ERR_UNSUPPORTED"
+    with pytest.raises(T.TcaError) as e:                             # E4M3 tiles take an E4M3 B
+        T.spmm(wide, wide.pack_dense(wide.pack(q)), dev_f16(case.B), dense=True)
     assert e.value.name == "TCA_ERR_UNSUPPORTED"
 
 
