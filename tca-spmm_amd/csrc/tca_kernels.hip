@@ -806,6 +806,19 @@ constexpr int block_lds_bytes()
     return cmax(KS * 2 * BlockLds<NT, WM, ES>::BTILE, (KS - 1) * WM * (NT / 16) * 1024);
 }
 
+// Row of the m-th row slot when a workgroup writes an 8-bit B tile (pitch
+// 8*SEGS + 16 bytes) in 8-byte pieces: the SEGS pieces of a row are
+// consecutive lanes, so a 256-byte write cycle covers 32 / SEGS rows, which
+// sit s rows apart (s = 4 for 64-byte rows, 8 for 128-byte rows) so that their
+// row spans tile the 64 banks exactly.  A permutation of the 32 rows.
+template <int SEGS> __device__ __forceinline__ int piece_row8(int m)
+{
+    constexpr int HW = 32 / SEGS;            // rows per write cycle
+    constexpr int S = SEGS == 8 ? 4 : 8;     // their spacing
+    const int h = m / HW, i = m % HW;
+    return h % S + S * HW * (h / S) + S * i;
+}
+
 // Transposing 8-bit LDS read (ds_read_b64_tr_b8, gfx950): in each 16-lane
 // group, lane 2r+j addresses row r, bytes 8j..8j+7 of an 8-row x 16-byte
 // block, and lane i receives byte i of the eight rows -- 8 consecutive k of
@@ -852,7 +865,7 @@ __device__ __forceinline__ void spmm_block_unit(const SpmmParams &p, int g, int 
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const int pc = tid + j * WM * 64;
-        pk[j] = ES == 2 ? piece_row<L::SEGS>(pc / L::SEGS) : pc / L::SEGS;
+        pk[j] = ES == 2 ? piece_row<L::SEGS>(pc / L::SEGS) : piece_row8<L::SEGS>(pc / L::SEGS);
         const int seg = pc % L::SEGS;
         const int ncol = n0 + 8 * seg;
         pok[j] = ncol < p.N;
