@@ -204,9 +204,10 @@ tca_status tca_spmm(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_til
  * transposed plan; same arguments.  Plans of 16 / 32 / 64 rows give the
  * results of tca_spmm bit for bit (the MFMAs see the same operands in the same
  * order); plans of 128 / 256 rows (F16 / BF16 A with an F16 / BF16 / F32 B, or
- * E4M3 A with an E4M3 B, tile_n 64 or 128) run the block kernels, whose two K
- * groups add their partial sums in a fixed order: the same results to FP32
- * summation order, the same bits from launch to launch. */
+ * E4M3 A with an E4M3 B, tile_n 64 or 128) run the block kernels, whose K
+ * groups (two per workgroup when the launch has fewer than 512 group x N-tile
+ * units, else one) add their partial sums in a fixed order: the same results
+ * to FP32 summation order, the same bits from launch to launch. */
 tca_status tca_spmm_dense(const tca_plan *plan, tca_dtype a_dtype, const void *d_a_dense,
                           tca_dtype b_dtype, const void *d_B, int ldb, int N,
                           float *d_C, int ldc, const float *d_row_scale, const float *d_col_scale,

@@ -1095,29 +1095,40 @@ SpmmFn pick_spmm(tca_dtype at, tca_dtype bt, int rpw, int nt)
 }
 
 // Block kernels on panel-dense tiles of 128 / 256-row plans: 4 waves per K
-// group (2 / 4 panels each), two K groups per workgroup.
-constexpr int kBlockWaves = 4, kBlockKS = 2;
+// group (2 / 4 panels each), ks = 1 or 2 K groups per workgroup: two when the
+// launch has fewer than kBlockKS2Units units (group x N-tile), so that each
+// CU still runs eight waves (4096^2 x 512, 256 units: 26.6 against 34.0 us
+// with one K group), one when there are enough units to put two or more
+// workgroups on every CU (4096x1024 x 2048, 512 units: 24.3 against 33.8 us
+// with two).
+constexpr int kBlockWaves = 4;
+constexpr long kBlockKS2Units = 512;   // two units per CU of the MI355X's 256
 
-template <tca_dtype AT, tca_dtype BT>
-SpmmFn pick_block(int rpw, int nt)
+template <tca_dtype AT, tca_dtype BT, int KS>
+SpmmFn pick_block_ks(int rpw, int nt)
 {
-    if (rpw == 128 && nt == 64) return spmm_block_kernel<2, 64, kBlockWaves, kBlockKS, AT, BT>;
-    if (rpw == 128 && nt == 128) return spmm_block_kernel<2, 128, kBlockWaves, kBlockKS, AT, BT>;
-    if (rpw == 256 && nt == 64) return spmm_block_kernel<4, 64, kBlockWaves, kBlockKS, AT, BT>;
-    if (rpw == 256 && nt == 128) return spmm_block_kernel<4, 128, kBlockWaves, kBlockKS, AT, BT>;
+    if (rpw == 128 && nt == 64) return spmm_block_kernel<2, 64, kBlockWaves, KS, AT, BT>;
+    if (rpw == 128 && nt == 128) return spmm_block_kernel<2, 128, kBlockWaves, KS, AT, BT>;
+    if (rpw == 256 && nt == 64) return spmm_block_kernel<4, 64, kBlockWaves, KS, AT, BT>;
+    if (rpw == 256 && nt == 128) return spmm_block_kernel<4, 128, kBlockWaves, KS, AT, BT>;
     return nullptr;
+}
+template <tca_dtype AT, tca_dtype BT>
+SpmmFn pick_block(int rpw, int nt, int ks)
+{
+    return ks == 2 ? pick_block_ks<AT, BT, 2>(rpw, nt) : pick_block_ks<AT, BT, 1>(rpw, nt);
 }
 
 // Panel-dense A (forward or transposed plans; 128 / 256 rows: F16 / BF16 A, or
 // E4M3 A and B, on the block kernels).
-SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt)
+SpmmFn pick_spmm_dense(tca_dtype at, tca_dtype bt, int rpw, int nt, int ks)
 {
     if (rpw >= 128) {
-        if (at == TCA_F16 && bt == TCA_F16) return pick_block<TCA_F16, TCA_F16>(rpw, nt);
-        if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_block<TCA_E4M3, TCA_E4M3>(rpw, nt);
-        if (at == TCA_BF16 && bt == TCA_BF16) return pick_block<TCA_BF16, TCA_BF16>(rpw, nt);
-        if (at == TCA_F16 && bt == TCA_F32) return pick_block<TCA_F16, TCA_F32>(rpw, nt);
-        if (at == TCA_BF16 && bt == TCA_F32) return pick_block<TCA_BF16, TCA_F32>(rpw, nt);
+        if (at == TCA_F16 && bt == TCA_F16) return pick_block<TCA_F16, TCA_F16>(rpw, nt, ks);
+        if (at == TCA_E4M3 && bt == TCA_E4M3) return pick_block<TCA_E4M3, TCA_E4M3>(rpw, nt, ks);
+        if (at == TCA_BF16 && bt == TCA_BF16) return pick_block<TCA_BF16, TCA_BF16>(rpw, nt, ks);
+        if (at == TCA_F16 && bt == TCA_F32) return pick_block<TCA_F16, TCA_F32>(rpw, nt, ks);
+        if (at == TCA_BF16 && bt == TCA_F32) return pick_block<TCA_BF16, TCA_F32>(rpw, nt, ks);
         return nullptr;
     }
     if (at == TCA_F16 && bt == TCA_F16) return pick_rpw<TCA_F16, TCA_F16, false, true>(rpw, nt);
@@ -1909,7 +1920,8 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     if (tile_n == 0) tile_n = default_tile_n(p, N);
     if (dense && dense_bytes(t, a_dtype) >= (1L << 31))
         return fail(TCA_ERR_UNSUPPORTED, "tca_spmm_dense: the dense tiles reach 2 GiB (32-bit offsets)");
-    SpmmFn fn = dense           ? pick_spmm_dense(a_dtype, b_dtype, t.rpw, tile_n)
+    const int block_ks = (long)t.n_groups * ((N + tile_n - 1) / tile_n) < kBlockKS2Units ? 2 : 1;
+    SpmmFn fn = dense           ? pick_spmm_dense(a_dtype, b_dtype, t.rpw, tile_n, block_ks)
                 : p->transposed ? pick_spmm<true>(a_dtype, b_dtype, t.rpw, tile_n)
                                 : pick_spmm<false>(a_dtype, b_dtype, t.rpw, tile_n);
     if (!fn) return fail(TCA_ERR_UNSUPPORTED, "tca_spmm: unsupported dtype pair / rows_per_wave / tile_n");
@@ -1944,9 +1956,10 @@ tca_status launch_spmm(const tca_plan *p, tca_dtype a_dtype, const void *a, tca_
     int W = 1, S = 1;
     size_t lds_bytes = 0;
     if (t.rpw >= 128 && dense) {
-        // block kernel: two K groups of four waves share each unit (no split:
-        // units that add atomically into C cost more than the parallelism gains)
-        W = kBlockWaves * kBlockKS;
+        // block kernel: one or two K groups of four waves share each unit (no
+        // split: units that add atomically into C cost more than the
+        // parallelism gains)
+        W = kBlockWaves * block_ks;
         lds_bytes = 0;
     } else if (t.rpw >= 128) {
         // wide groups: W = rows / 32 waves share each B tile, one unit per group

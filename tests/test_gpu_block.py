@@ -5,7 +5,9 @@ tile and two K groups split the chunk range.  Checked against the FP64 oracle
 tiled-values launch of the same plan (same products, another summation order:
 rel-L2 <= 1e-6): forward and transposed plans, F16 / BF16 / an F32 B rounded on
 load, tile widths 64 / 128, ragged N and row counts, empty and one-row groups,
-groups of one chunk (one K group idle), and the 4096 x 4096 x 512 workload;
+groups of one chunk (one K group idle), the 4096 x 4096 x 512 workload (two
+K groups per workgroup) and the 4096 x 1024 x 2048 one (one K group: enough
+units to fill the chip);
 E4M3 A and B with per-row / per-column scales (8-bit tiles, B operands by
 transposing 8-bit LDS reads) against the oracle on the quantised inputs and
 scales."""
@@ -160,6 +162,29 @@ def test_block_e4m3_4096_workload_and_transposed(cuda):
         tr = fwd.transpose(rpw)
         got = T.spmm(tr, tr.pack_dense(fwd.pack(qw)), gq, tile_n=64, dense=True).cpu().numpy()
         assert rel_l2(got, ref_t) <= 1e-5, rpw
+
+
+def test_block_one_k_group_when_units_fill_the_chip(cuda):
+    # launches with >= 512 units (group x N-tile) run one K group per
+    # workgroup: the 4096 x 1024 x 2048 layer (128-row plans: 1024 / 512
+    # units; 256-row plans: 512 with tile 64, 256 -- two K groups -- with
+    # tile 128), FP16 and E4M3, against the FP64 oracle
+    case = workloads.sparse_linear(seed=0)
+    A, N = case.A, case.B.shape[1]
+    ref = O.spmm_f64(A.n_rows, N, A.row_offset, A.col_idx, f16_to_f64(A.values), f16_to_f64(case.B))
+    B = dev_f16(case.B)
+    q, s = T.quantize_rows_e4m3(dev(A.row_offset), dev_f16(A.values))
+    qb, sb = T.quantize_cols_e4m3(B)
+    ref8 = _e4m3_ref(A, q, s, qb, sb, N)
+    for rpw in WIDE:
+        plan = T.Plan(A.n_rows, A.n_cols, A.row_offset, A.col_idx, rpw)
+        d = plan.pack_dense(plan.pack(dev_f16(A.values)))
+        d8 = plan.pack_dense(plan.pack(q))
+        for t in TILES:
+            got = T.spmm(plan, d, B, tile_n=t, dense=True).cpu().numpy()
+            assert rel_l2(got, ref) <= 1e-5, (rpw, t)
+            got8 = T.spmm(plan, d8, qb, row_scale=s, col_scale=sb, tile_n=t, dense=True).cpu().numpy()
+            assert rel_l2(got8, ref8) <= 1e-5, (rpw, t)
 
 
 @pytest.mark.parametrize("rpw", (64, 128))
