@@ -295,6 +295,8 @@ def main(argv=None):
     ap.add_argument("--batch", action="store_true", help="also tune the batched training step of batch12")
     ap.add_argument("--batch-only", action="store_true", help="only tune the batched training step (keeps the table)")
     ap.add_argument("--passes", type=int, default=1, help="coordinate-descent passes of the batch tuning")
+    ap.add_argument("--update", action="store_true",
+                    help="start from the current table and replace only the entries tuned now")
     args = ap.parse_args(argv)
     global _table
     dev = torch.device("cuda:0")
@@ -305,7 +307,11 @@ def main(argv=None):
         ch, us = tune_batch(cases, passes=args.passes, log=lambda m: print(m, flush=True))
         record_batch(cases, ch, us, path=args.out)
         return
-    _table = {"entries": []}
+    if args.update:
+        _table = None
+        _load()
+    else:
+        _table = {"entries": []}
 
     def f16(bits):
         return torch.from_numpy(np.ascontiguousarray(bits).view(np.int16)).view(torch.float16).to(dev)
