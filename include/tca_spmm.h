@@ -301,11 +301,15 @@ tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t
 /* transposed, with an F16 B on 16-row (tile_n 16/32/64), 32-row (32/64),    */
 /* 64-row (32/64) or 128-row (64) plans, or an F32 B on 16-row (16/32/64) or  */
 /* 32-row (32) plans.                                                         */
-/* SDDMM items: F16 X, F32 or F16 G.  Other combinations: UNSUPPORTED.       */
-/* An F32 operand (SpMM B, SDDMM G) gets a batch-owned F16 copy (2 bytes per  */
-/* element) that every run refreshes first -- same bits as rounding on load -- */
-/* and the items read the copy: half the gather bytes.                       */
-/* a_dense = 2 (SPMM, F16): `a` holds the weight's VALUES, in the CSR order  */
+/* SDDMM items: F16 X, F32 or F16 G.                                          */
+/* BF16: the panel-dense SPMM items above with BF16 A (and a BF16 or F32 B),  */
+/* SDDMM items with BF16 X (and a BF16 or F32 G); a batch takes F16 or BF16   */
+/* items, not both.  Other combinations: UNSUPPORTED.                         */
+/* An F32 operand (SpMM B, SDDMM G) gets a batch-owned 16-bit copy (2 bytes  */
+/* per element, in the type its readers round it to: FP16 or BF16) that every */
+/* run refreshes first -- same bits as rounding on load -- and the items read */
+/* the copy: half the gather bytes.                                           */
+/* a_dense = 2 (SPMM, F16/BF16): `a` holds the weight's VALUES, in CSR order */
 /* of the forward matrix (values_order NULL) or in the tiled order of          */
 /* values_order (a forward plan of the same pattern, e.g. the SDDMM item's    */
 /* plan, so that dW and the weight share one order); the batch owns the        */

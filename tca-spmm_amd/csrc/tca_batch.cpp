@@ -10,7 +10,7 @@ using tca::fail;
 
 struct tca_batch {
     int n_items = 0, n_units = 0, grid = 0, device = 0, n_zero = 0;
-    bool dense_only = false;   // every SpMM item has panel-dense tiles: the leaner kernel
+    int mode = 0;   // 1: every item F16 and every SpMM item panel-dense (the leaner kernel); 2: the same in BF16
     void *d_mem = nullptr;  // items | units | zero list | counters
     tca::BatchItemDev *d_items = nullptr;
     tca::BatchUnitDev *d_units = nullptr;
@@ -45,24 +45,27 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
 {
     if (!batch || n_items <= 0 || !items) return fail(TCA_ERR_INVALID_ARG, "tca_batch_create: bad argument");
     *batch = nullptr;
-    // FP32 operands read by several items or one: one FP16 copy per buffer, used
-    // when every item reading it agrees on its shape
+    // FP32 operands read by several items or one: one 16-bit copy per buffer, in
+    // the type the reading items round it to on load (an SPMM item's A type, an
+    // SDDMM item's X type: FP16 or BF16), used when every item reading it
+    // agrees on its shape and that type
     std::vector<tca_batch_item> its(items, items + n_items);
     struct Operand {
         const void *src;
         int rows, N, ld;
         bool ok;
         size_t off;
+        tca_dtype to;
     };
     std::vector<Operand> ops;
     auto operand_of = [&](int i, bool is_b) -> std::pair<const void *, Operand> {
         const tca_batch_item &it = its[i];
         const tca_plan *p = it.plan;
         if (!p) return {nullptr, {}};
-        if (it.op == TCA_OP_SPMM && is_b && it.b_dtype == TCA_F32 && it.b)
-            return {it.b, {it.b, p->host.n_cols, it.N, it.ldb, true, 0}};
-        if (it.op == TCA_OP_SDDMM && !is_b && it.a_dtype == TCA_F32 && it.a)
-            return {it.a, {it.a, p->host.n_rows, it.N, it.lda, true, 0}};
+        if (it.op == TCA_OP_SPMM && is_b && it.b_dtype == TCA_F32 && it.b && (it.a_dtype == TCA_F16 || it.a_dtype == TCA_BF16))
+            return {it.b, {it.b, p->host.n_cols, it.N, it.ldb, true, 0, it.a_dtype}};
+        if (it.op == TCA_OP_SDDMM && !is_b && it.a_dtype == TCA_F32 && it.a && (it.b_dtype == TCA_F16 || it.b_dtype == TCA_BF16))
+            return {it.a, {it.a, p->host.n_rows, it.N, it.lda, true, 0, it.b_dtype}};
         return {nullptr, {}};
     };
     for (int i = 0; i < n_items; i++)
@@ -73,7 +76,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             if (!ptr || o.N <= 0 || o.N % 8 || o.ld < o.N || o.ld % 8 || o.rows <= 0) continue;
             auto f = std::find_if(ops.begin(), ops.end(), [&](const Operand &x) { return x.src == ptr; });
             if (f == ops.end()) ops.push_back(o);
-            else if (f->rows != o.rows || f->N != o.N || f->ld != o.ld) f->ok = false;
+            else if (f->rows != o.rows || f->N != o.N || f->ld != o.ld || f->to != o.to) f->ok = false;
         }
     ops.erase(std::remove_if(ops.begin(), ops.end(), [](const Operand &x) { return !x.ok; }), ops.end());
     if (ops.size() > (size_t)tca::kMaxConv) ops.resize(tca::kMaxConv);   // the rest stay FP32
@@ -102,21 +105,23 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             c.rows = o.rows;
             c.N = o.N;
             c.ld = o.ld;
+            c.bf16 = o.to == TCA_BF16 ? 1 : 0;
             c.first8 = conv8;
             conv8 += ((long)o.rows * o.N / 8 + 255) / 256 * 256;   // next operand on a workgroup boundary
             conv.push_back(c);
         }
         for (tca_batch_item &it : its) {
             for (const tca::ConvDesc &c : conv) {
+                const tca_dtype to = c.bf16 ? TCA_BF16 : TCA_F16;
                 if (it.op == TCA_OP_SPMM && it.b_dtype == TCA_F32 && it.b == c.src) {
                     it.b = c.dst;
                     it.ldb = c.N;
-                    it.b_dtype = TCA_F16;
+                    it.b_dtype = to;
                 }
                 if (it.op == TCA_OP_SDDMM && it.a_dtype == TCA_F32 && it.a == c.src) {
                     it.a = c.dst;
                     it.lda = c.N;
-                    it.a_dtype = TCA_F16;
+                    it.a_dtype = to;
                 }
             }
         }
@@ -140,7 +145,8 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
         const std::string who = "tca_batch_create: item " + std::to_string(i);
         if (it.op != TCA_OP_SPMM || !p || !p->d_base)
             return fail(TCA_ERR_INVALID_ARG, who + ": a_dense = 2 needs an SPMM item with a device plan");
-        if (it.a_dtype != TCA_F16) return fail(TCA_ERR_UNSUPPORTED, who + ": a_dense = 2 needs F16 values");
+        if (it.a_dtype != TCA_F16 && it.a_dtype != TCA_BF16)
+            return fail(TCA_ERR_UNSUPPORTED, who + ": a_dense = 2 needs F16 or BF16 values");
         const tca_plan *q = it.values_order;
         if (q && (q->transposed || q->pattern_key != p->pattern_key || q->host.nnz != p->host.nnz))
             return fail(TCA_ERR_INVALID_ARG, who + ": values_order must be a forward plan of the item's pattern");
@@ -269,10 +275,10 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) +
                                                      " (batched SpMM: f16 A with f16 B forward on rows_per_wave 16 with tile_n "
                                                      "16/32/64 or rows_per_wave 32 with tile_n 32; f16 A with f32 or f16 B "
-                                                     "transposed on rows_per_wave 16 with tile_n 16/32/64; panel-dense f16 A "
-                                                     "with f16 B on rows_per_wave 16 (tile_n 16/32/64), 32 (32/64), "
-                                                     "64 (32/64) or 128 (64), or f32 B on rows_per_wave 16 (16/32/64) "
-                                                     "or 32 (32))");
+                                                     "transposed on rows_per_wave 16 with tile_n 16/32/64; panel-dense f16 "
+                                                     "or bf16 A with a B of the same type on rows_per_wave 16 (tile_n "
+                                                     "16/32/64), 32 (32/64), 64 (32/64) or 128 (64), or f32 B on "
+                                                     "rows_per_wave 16 (16/32/64) or 32 (32))");
             if (it.a_dense && tca::dense_bytes(t, it.a_dtype) >= (1L << 31))
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SPMM item " + std::to_string(i) + " dense tiles reach 2 GiB");
             d.ldb = it.ldb;
@@ -290,7 +296,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
                     // FP32 B gathers twice the bytes; tiled values add the entry work)
                     // (block units: the four waves share each chunk, about half the
                     // cost per panel of a wave per 32 rows)
-                    const bool block = d.variant == 22 || d.variant == 23;
+                    const bool block = d.variant == 22 || d.variant == 23 || d.variant == 34 || d.variant == 35;
                     const double chunk = (100.0 + (it.b_dtype == TCA_F32 ? 280.0 : 150.0) * ppg * tile / 64.0) *
                                          (it.a_dense ? 1.0 : 1.6) * (block ? 0.5 : 1.0);
                     cost.push_back(1500.0 + nch * chunk);
@@ -304,7 +310,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
             d.variant = tca::batch_variant(TCA_OP_SDDMM, t.rpw, 0, it.a_dtype, it.b_dtype, false, false);
             if (d.variant < 0)
                 return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: SDDMM item " + std::to_string(i) +
-                                                     " (batched SDDMM: f16 X, f32 or f16 G)");
+                                                     " (batched SDDMM: f16 X with f32 or f16 G, bf16 X with f32 or bf16 G)");
             // the SDDMM runs on the plan's 64-row tiling and writes through dst
             const tca_plan *aux = nullptr;
             const int32_t *dst = nullptr;
@@ -354,10 +360,16 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     int dev_id = 0, cus = 0;
     TCA_HIP_TRY(hipGetDevice(&dev_id));
     TCA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
-    bool dense_only = true;
-    for (int i = 0; i < n_items; i++)
+    bool dense_only = true, any_bf16 = false, any_f16 = false;
+    for (int i = 0; i < n_items; i++) {
         if (items[i].op == TCA_OP_SPMM && !items[i].a_dense) dense_only = false;
-    const int workers = std::max(cus, 1) * tca::batch_blocks_per_cu(dense_only);
+        const tca_dtype t16 = items[i].op == TCA_OP_SPMM ? items[i].a_dtype : items[i].b_dtype;   // A / X type
+        (t16 == TCA_BF16 ? any_bf16 : any_f16) = true;
+    }
+    if (any_bf16 && any_f16)
+        return fail(TCA_ERR_UNSUPPORTED, "tca_batch_create: a batch takes F16 or BF16 items, not both");
+    const int mode = any_bf16 ? 2 : dense_only ? 1 : 0;
+    const int workers = std::max(cus, 1) * tca::batch_blocks_per_cu(mode);
     double total_cost = 0.0;
     for (double c : cost) total_cost += c;
     const double share = total_cost / workers;
@@ -443,7 +455,7 @@ extern "C" tca_status tca_batch_create(int n_items, const tca_batch_item *items,
     b->n_units = (int)sorted.size();
     b->n_zero = (int)zero.size();
     b->device = dev_id;
-    b->dense_only = dense_only;
+    b->mode = mode;
     b->grid = grid;
     b->nparts = nparts;
     b->n_conv = (int)conv.size();
@@ -497,7 +509,7 @@ extern "C" tca_status tca_batch_run_stage(tca_batch *batch, int stage, void *str
     }
     if (!(stage & 2)) return TCA_OK;
     return tca::launch_batch(batch->d_items, batch->d_units, batch->n_units, batch->d_counters, batch->d_part_off,
-                             batch->nparts, batch->grid, batch->d_zero, batch->n_zero, batch->dense_only,
+                             batch->nparts, batch->grid, batch->d_zero, batch->n_zero, batch->mode,
                              static_cast<hipStream_t>(stream));
 }
 

@@ -157,10 +157,10 @@ struct ZeroDesc {              // an output that split units accumulate into
     float *C;
     int32_t rows, N, ldc, pad;
 };
-struct ConvDesc {              // an FP32 operand a batch rounds to an FP16 copy on every run
+struct ConvDesc {              // an FP32 operand a batch rounds to an FP16 / BF16 copy on every run
     const float *src;
     uint16_t *dst;             // rows x N, row pitch N
-    int32_t rows, N, ld, pad;
+    int32_t rows, N, ld, bf16; // bf16: the copy is BF16 (else FP16)
     int64_t first8;            // its first 8-element group in the conversion launch (a multiple of 256)
 };
 // FP16 copies: every operand of a ConvDesc list is rounded (nearest even, the
@@ -194,9 +194,10 @@ constexpr size_t kBatchCounterBytes = (size_t)(kBatchParts + 1) * kCounterStride
 // -1 when the combination has no batched variant.
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense);
 // resident one-wave workgroups of batch_kernel per compute unit
-int batch_blocks_per_cu(bool dense_only);
+// mode: 0 = every F16 case, 1 = F16 panel-dense SpMM + SDDMM, 2 = the BF16 ones
+int batch_blocks_per_cu(int mode);
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters,
-                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, bool dense_only,
+                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, int mode,
                         hipStream_t stream);
 
 }  // namespace tca

@@ -1577,7 +1577,9 @@ __global__ void warm_up_kernel() {}
 // (15..17) and 32-row plans with tile_n 32 (21); block units (the workgroup's
 // four waves share each B chunk tile) on 64-row (22) and 128-row (23) plans
 // with tile_n 64 and an f16 B; SDDMM with an f32 (5) or f16 (6) G on the plan's
-// 64-row SDDMM tiling.
+// 64-row SDDMM tiling.  BF16: the panel-dense cases with BF16 operands (24..35,
+// each its F16 counterpart + 12) and the SDDMM with a bf16 X and an f32 (36) or
+// bf16 (37) G.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ SpmmParams to_spmm(const BatchItemDev &d)
 {
@@ -1647,11 +1649,14 @@ constexpr int kBatchSlot = cmax(cmax(SpmmLds<16, 64, TCA_F16>::SLOT, SpmmLds<32,
 constexpr int kBatchLds = cmax(cmax(kBatchWaves * kBatchSlot, ShLds<kBatchSddmmCB>::BYTES),
                                 cmax(block_lds_bytes<1, 64, kBatchWaves, 1>(), block_lds_bytes<2, 64, kBatchWaves, 1>()));
 
-// DENSE_ONLY: the SpMM cases of panel-dense items only (and the SDDMM); the
-// register budget is then that of the lean dense loop, which keeps four
-// workgroups (16 waves) on each CU instead of three.
-template <bool DENSE_ONLY>
-__global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(DENSE_ONLY ? 4 : 3))) void batch_kernel(const BatchItemDev *__restrict__ items,
+// MODE 0: every F16 case; 1 (DENSE_ONLY): the F16 SpMM cases of panel-dense
+// items only (and the F16 SDDMM) -- the register budget is then that of the
+// lean dense loop, which keeps four workgroups (16 waves) on each CU instead
+// of three; 2: the same cases for BF16 operands (a batch of BF16 items, all
+// panel-dense), an instantiation of its own so that neither type's cases
+// weigh on the other's register budget.
+template <int MODE>
+__global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(MODE ? 4 : 3))) void batch_kernel(const BatchItemDev *__restrict__ items,
                                                                  const BatchUnitDev *__restrict__ units, int n_units,
                                                                  int *counters, const int *__restrict__ part_off,
                                                                  int nparts)
@@ -1699,30 +1704,45 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
         const BatchUnitDev un = units[u];
         const BatchItemDev d = items[un.item];
         switch (d.variant) {
-        case 0: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 1: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 2: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 3: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 4: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 7: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 8: if constexpr (!DENSE_ONLY) spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 9: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 10: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 11: if constexpr (!DENSE_ONLY) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 12: spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 13: spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 14: spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 15: spmm_wg_unit<16, 16, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 16: spmm_wg_unit<16, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 17: spmm_wg_unit<16, 64, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 18: spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 19: spmm_wg_unit<32, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 20: spmm_wg_unit<64, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 21: spmm_wg_unit<32, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
-        case 22: spmm_block_unit<1, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
-        case 23: spmm_block_unit<2, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
-        case 5: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
-        case 6: sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
+        case 0: if constexpr (MODE == 0) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 1: if constexpr (MODE == 0) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 2: if constexpr (MODE == 0) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 3: if constexpr (MODE == 0) spmm_wg_unit<16, 16, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 4: if constexpr (MODE == 0) spmm_wg_unit<16, 32, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 7: if constexpr (MODE == 0) spmm_wg_unit<16, 64, TCA_F16, TCA_F32, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 8: if constexpr (MODE == 0) spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 9: if constexpr (MODE == 0) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 10: if constexpr (MODE == 0) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 11: if constexpr (MODE == 0) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 12: if constexpr (MODE != 2) spmm_wg_unit<16, 16, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 13: if constexpr (MODE != 2) spmm_wg_unit<16, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 14: if constexpr (MODE != 2) spmm_wg_unit<16, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 15: if constexpr (MODE != 2) spmm_wg_unit<16, 16, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 16: if constexpr (MODE != 2) spmm_wg_unit<16, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 17: if constexpr (MODE != 2) spmm_wg_unit<16, 64, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 18: if constexpr (MODE != 2) spmm_wg_unit<32, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 19: if constexpr (MODE != 2) spmm_wg_unit<32, 64, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 20: if constexpr (MODE != 2) spmm_wg_unit<64, 32, TCA_F16, TCA_F16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 21: if constexpr (MODE != 2) spmm_wg_unit<32, 32, TCA_F16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 22: if constexpr (MODE != 2) spmm_block_unit<1, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
+        case 23: if constexpr (MODE != 2) spmm_block_unit<2, 64, kBatchWaves, 1, 2, TCA_F16, TCA_F16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
+        case 5: if constexpr (MODE != 2) sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
+        case 6: if constexpr (MODE != 2) sddmm_shared_unit<kBatchSddmmCB, TCA_F16, TCA_F16>(to_sddmm(d), un.g, un.t, smem); break;
+        // BF16: the panel-dense and SDDMM cases above with BF16 operands
+        case 24: if constexpr (MODE == 2) spmm_wg_unit<16, 16, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 25: if constexpr (MODE == 2) spmm_wg_unit<16, 32, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 26: if constexpr (MODE == 2) spmm_wg_unit<16, 64, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 27: if constexpr (MODE == 2) spmm_wg_unit<16, 16, TCA_BF16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 28: if constexpr (MODE == 2) spmm_wg_unit<16, 32, TCA_BF16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 29: if constexpr (MODE == 2) spmm_wg_unit<16, 64, TCA_BF16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 30: if constexpr (MODE == 2) spmm_wg_unit<32, 32, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 31: if constexpr (MODE == 2) spmm_wg_unit<32, 64, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 32: if constexpr (MODE == 2) spmm_wg_unit<64, 32, TCA_BF16, TCA_BF16, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 33: if constexpr (MODE == 2) spmm_wg_unit<32, 32, TCA_BF16, TCA_F32, false, true>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem, kBatchSlot); break;
+        case 34: if constexpr (MODE == 2) spmm_block_unit<1, 64, kBatchWaves, 1, 2, TCA_BF16, TCA_BF16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
+        case 35: if constexpr (MODE == 2) spmm_block_unit<2, 64, kBatchWaves, 1, 2, TCA_BF16, TCA_BF16>(to_spmm(d), un.g, un.t, un.c_lo, un.c_hi, un.atomic != 0, smem); break;
+        case 36: if constexpr (MODE == 2) sddmm_shared_unit<kBatchSddmmCB, TCA_BF16, TCA_F32>(to_sddmm(d), un.g, un.t, smem); break;
+        case 37: if constexpr (MODE == 2) sddmm_shared_unit<kBatchSddmmCB, TCA_BF16, TCA_BF16>(to_sddmm(d), un.g, un.t, smem); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next unit
@@ -1754,20 +1774,22 @@ __global__ void zero_list_kernel(const ZeroDesc *__restrict__ list)
 int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b_dtype, bool transposed, bool dense)
 {
     if (dense) {   // panel-dense tiles, forward or transposed plan alike
-        if (op != TCA_OP_SPMM || a_dtype != TCA_F16 || (b_dtype != TCA_F16 && b_dtype != TCA_F32)) return -1;
+        if (op != TCA_OP_SPMM || (a_dtype != TCA_F16 && a_dtype != TCA_BF16) || (b_dtype != a_dtype && b_dtype != TCA_F32))
+            return -1;
         const int f32 = b_dtype == TCA_F32;
+        const int bf = a_dtype == TCA_BF16 ? 12 : 0;   // BF16: the F16 case + 12 (24..35)
         if (rpw == 16) {
             const int t = tile_n == 16 ? 0 : tile_n == 32 ? 1 : tile_n == 64 ? 2 : -1;
-            return t < 0 ? -1 : (f32 ? 15 : 12) + t;
+            return t < 0 ? -1 : (f32 ? 15 : 12) + t + bf;
         }
         // (F32 B on wider panels only at rows_per_wave 32 x tile_n 32: the others
         // would push the dense kernel past four workgroups per CU)
-        if (rpw == 32 && tile_n == 32) return f32 ? 21 : 18;
-        if (!f32 && rpw == 32 && tile_n == 64) return 19;
-        if (!f32 && rpw == 64 && tile_n == 32) return 20;
+        if (rpw == 32 && tile_n == 32) return (f32 ? 21 : 18) + bf;
+        if (!f32 && rpw == 32 && tile_n == 64) return 19 + bf;
+        if (!f32 && rpw == 64 && tile_n == 32) return 20 + bf;
         // block units: the workgroup's four waves share each B chunk tile
-        if (!f32 && rpw == 64 && tile_n == 64) return 22;
-        if (!f32 && rpw == 128 && tile_n == 64) return 23;
+        if (!f32 && rpw == 64 && tile_n == 64) return 22 + bf;
+        if (!f32 && rpw == 128 && tile_n == 64) return 23 + bf;
         return -1;
     }
     if (op == TCA_OP_SPMM) {
@@ -1784,20 +1806,24 @@ int batch_variant(tca_op op, int rpw, int tile_n, tca_dtype a_dtype, tca_dtype b
         if (a_dtype == TCA_F32) return 5;
         if (a_dtype == TCA_F16) return 6;
     }
+    if (op == TCA_OP_SDDMM && b_dtype == TCA_BF16 && !transposed) {
+        if (a_dtype == TCA_F32) return 36;
+        if (a_dtype == TCA_BF16) return 37;
+    }
     return -1;
 }
 
-int batch_blocks_per_cu(bool dense_only)
+int batch_blocks_per_cu(int mode)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dense_only ? batch_kernel<true> : batch_kernel<false>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mode == 2 ? batch_kernel<2> : mode == 1 ? batch_kernel<1> : batch_kernel<0>,
                                                      64 * kBatchWaves, 0) != hipSuccess || n < 1)
         n = 1;
     return n;
 }
 
 tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, int n_units, int *counters,
-                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, bool dense_only,
+                        const int *part_off, int nparts, int grid, const ZeroDesc *zero, int n_zero, int mode,
                         hipStream_t stream)
 {
     if (n_zero > 0) {
@@ -1806,11 +1832,14 @@ tca_status launch_batch(const BatchItemDev *items, const BatchUnitDev *units, in
     }
     if (n_units <= 0) return TCA_OK;
     if (nparts < 1 || nparts > kBatchParts || nparts > grid) return fail(TCA_ERR_INVALID_ARG, "batch: partitions");
-    if (dense_only)
-        hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
+    if (mode == 2)
+        hipLaunchKernelGGL(batch_kernel<2>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
+                           counters, part_off, nparts);
+    else if (mode == 1)
+        hipLaunchKernelGGL(batch_kernel<1>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
                            counters, part_off, nparts);
     else
-        hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
+        hipLaunchKernelGGL(batch_kernel<0>, dim3(grid), dim3(64 * kBatchWaves), 0, stream, items, units, n_units,
                            counters, part_off, nparts);
     TCA_HIP_TRY(hipGetLastError());
     return TCA_OK;
@@ -2127,7 +2156,11 @@ __global__ __launch_bounds__(256) void prologue_kernel(const ConvDesc *conv, int
         const long r = e / d.N, col = e - r * d.N;
         const u32x4 *src = reinterpret_cast<const u32x4 *>(d.src + r * d.ld + col);
         const u32x4 a = gbl(src)[0], b = gbl(src)[1];
-        auto h = [](uint32_t x) { return f32_to_f16_bits(__builtin_bit_cast(float, x)); };
+        const bool bf = d.bf16 != 0;
+        auto h = [bf](uint32_t x) {
+            const float f = __builtin_bit_cast(float, x);
+            return bf ? f32_to_bf16_bits(f) : f32_to_f16_bits(f);
+        };
         const u32x4 o = {h(a[0]) | h(a[1]) << 16, h(a[2]) | h(a[3]) << 16, h(b[0]) | h(b[1]) << 16, h(b[2]) | h(b[3]) << 16};
         gbl(reinterpret_cast<u32x4 *>(d.dst + r * d.N + col))[0] = o;
         return;

@@ -15,10 +15,10 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import dev, dev_f16, f16_to_f64, rel_l2
+from helpers import dev, dev_bf16, dev_f16, f16_to_f64, rel_l2
 from oracle import oracle as O
 import tca_spmm as T
-from tca_spmm import workloads
+from tca_spmm import gen, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -215,6 +215,60 @@ def test_fused_sgd_bf16_weights_and_rejections(cuda):
         T.Batch([T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"], sgd=(big[:W.nnz], 0.5)),
                  T.sddmm_item(L["p16"], L["G"], L["X"], dW2, sgd=(big[8:W.nnz + 8], 0.5))])
     assert e.value.name == "TCA_ERR_INVALID_ARG"
+
+
+def test_train_step_bf16(cuda):
+    # the same training step in BF16: BF16 weights (tiled order) and
+    # activations, FP32 G rounded once per run into a batch-owned BF16 copy,
+    # the fused update on BF16 weights; against the FP64 oracle on the BF16
+    # inputs, and the weights against a separate tca_sgd_update bit for bit
+    lr = 0.0625
+    cases = workloads.batch12(seed=0)
+    layers, items = [], []
+    for i in (0, 4, 9, 11):
+        c = cases[i]
+        W, N = c.A, c.N
+        wb = gen.f32_to_bf16_bits(W.values.view(np.float16).astype(np.float32))
+        xb = gen.f32_to_bf16_bits(c.B.view(np.float16).astype(np.float32))
+        p16 = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, 16)
+        rpw, tile = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)
+        rpw_t, tile_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True)
+        pf = p16 if rpw == 16 else T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, rpw)
+        pt = p16.transpose(rpw_t)
+        L = dict(case=c, p16=p16, w=p16.pack(dev_bf16(wb)), X=dev_bf16(xb), G=dev(c.extra["G"]), xb=xb,
+                 Y=torch.full((W.n_rows, N), float("nan"), device=cuda),
+                 dX=torch.full((W.n_cols, N), float("nan"), device=cuda),
+                 dW=torch.full((W.nnz,), float("nan"), device=cuda))
+        items += [T.spmm_item(pf, L["w"], L["X"], L["Y"], tile, dense="pack", values_order=p16),
+                  T.spmm_item(pt, L["w"], L["G"], L["dX"], tile_t, dense="pack", values_order=p16),
+                  T.sddmm_item(p16, L["G"], L["X"], L["dW"], sgd=(L["w"], lr))]
+        layers.append(L)
+    bt = T.Batch(items)
+    for run in range(2):
+        before = [L["w"].clone() for L in layers]
+        bt.run()
+        torch.cuda.synchronize()
+        for L, w0 in zip(layers, before):
+            W, N = L["case"].A, L["case"].N
+            a64 = O.bf16_bits_to_f32(L["p16"].unpack(w0).view(torch.int16).cpu().numpy().view(np.uint16)).astype(np.float64)
+            x64 = O.bf16_bits_to_f32(L["xb"]).astype(np.float64)
+            g64 = O.bf16_bits_to_f32(O.f32_to_bf16_bits(L["case"].extra["G"])).astype(np.float64)
+            rpT, cT, perm = O.transpose_csr(W.n_rows, W.n_cols, W.row_offset, W.col_idx)
+            errs = (rel_l2(L["Y"].cpu().numpy(), O.spmm_f64(W.n_rows, N, W.row_offset, W.col_idx, a64, x64)),
+                    rel_l2(L["dX"].cpu().numpy(), O.spmm_f64(W.n_cols, N, rpT, cT, a64[perm], g64)),
+                    rel_l2(L["p16"].unpack(L["dW"]).cpu().numpy(), O.sddmm_f64(W.n_rows, N, W.row_offset, W.col_idx, g64, x64)))
+            assert max(errs) <= TOL, (W.name, run, errs)
+            want = w0.clone()
+            T.sgd_update(want, L["dW"], lr)
+            torch.cuda.synchronize()
+            assert torch.equal(L["w"].view(torch.int16), want.view(torch.int16)), (W.name, run)
+    # F16 and BF16 items in one batch: rejected
+    L = layers[0]
+    w16 = torch.zeros_like(L["w"], dtype=torch.float16)
+    with pytest.raises(T.TcaError) as e:
+        T.Batch([T.spmm_item(L["p16"], w16, L["X"].to(torch.float16), L["Y"], 64, dense="pack", values_order=L["p16"]),
+                 T.sddmm_item(L["p16"], L["G"], L["X"], L["dW"])])
+    assert e.value.name == "TCA_ERR_UNSUPPORTED"
 
 
 def test_train_step_rejects_foreign_values_order(cuda):
