@@ -380,7 +380,61 @@ def bench_train_step(torch, np, T, O, dev, args):
             lib_dx = lib_dw = float("nan")
         res["rocsparse_dx_ms"] = lib_dx
         res["rocsparse_dw_ms"] = lib_dw
+    res["bf16_step"] = bench_train_step_bf16(torch, np, T, O, dev, K)
     return ts, res
+
+
+def bench_train_step_bf16(torch, np, T, O, dev, K):
+    """The same twelve-layer training step in BF16 (weights and activations
+    BF16, G FP32 copied to BF16 by the batch, the fused update on BF16
+    weights): one batch run per step, timed like the FP16 step; the first
+    run's outputs of two layers against the FP64 oracle on the BF16 inputs."""
+    from tca_spmm import gen, workloads
+    cases = workloads.batch12(seed=0)
+    items, keep, check = [], [], []
+    for i, c in enumerate(cases):
+        W, N = c.A, c.N
+        wb = gen.f32_to_bf16_bits(W.values.view(np.float16).astype(np.float32))
+        xb = gen.f32_to_bf16_bits(c.B.view(np.float16).astype(np.float32))
+        r, t = T.autotune.choice_for(W.n_rows, W.n_cols, W.nnz, N, batch=True, dense=True)
+        r_t, t_t = T.autotune.choice_for(W.n_cols, W.n_rows, W.nnz, N, op="spmm_t", batch=True, dense=True)
+        plan = T.Plan(W.n_rows, W.n_cols, W.row_offset, W.col_idx, r)
+        plan_t = plan.transpose(r_t)
+        w = plan.pack(torch.from_numpy(wb.view(np.int16)).view(torch.bfloat16).to(dev))
+        X = torch.from_numpy(xb.view(np.int16)).view(torch.bfloat16).to(dev)
+        G = torch.from_numpy(c.extra["G"]).to(dev)
+        Y = torch.empty((W.n_rows, N), dtype=torch.float32, device=dev)
+        dX = torch.empty((W.n_cols, N), dtype=torch.float32, device=dev)
+        dW = torch.empty(max(W.nnz, 1), dtype=torch.float32, device=dev)
+        items += [T.spmm_item(plan, w, X, Y, t, dense="pack", values_order=plan),
+                  T.spmm_item(plan_t, w, G, dX, t_t, dense="pack", values_order=plan),
+                  T.sddmm_item(plan, G, X, dW, sgd=(w, SGD_LR))]
+        keep += [plan, plan_t, w, X, G, Y, dX, dW]
+        if i in (0, len(cases) - 1):
+            check.append((c, plan, wb, xb, Y, dX, dW))
+    batch = T.Batch(items)
+    batch.run()
+    torch.cuda.synchronize()
+    errs = {}
+    for c, plan, wb, xb, Y, dX, dW in check:
+        W, N = c.A, c.N
+        a64 = O.bf16_bits_to_f32(wb).astype(np.float64)
+        x64 = O.bf16_bits_to_f32(xb).astype(np.float64)
+        g64 = O.bf16_bits_to_f32(O.f32_to_bf16_bits(c.extra["G"])).astype(np.float64)
+        rpT, cT, perm = O.transpose_csr(W.n_rows, W.n_cols, W.row_offset, W.col_idx)
+        e = [rel_l2(np, Y.cpu().numpy(), O.spmm_f64(W.n_rows, N, W.row_offset, W.col_idx, a64, x64)),
+             rel_l2(np, dX.cpu().numpy(), O.spmm_f64(W.n_cols, N, rpT, cT, a64[perm], g64)),
+             rel_l2(np, plan.unpack(dW).cpu().numpy(), O.sddmm_f64(W.n_rows, N, W.row_offset, W.col_idx, g64, x64))]
+        if max(e) > 1e-5:
+            raise RuntimeError(f"{W.name}: BF16 step vs FP64 oracle {e}")
+        errs[W.name] = {"fwd": e[0], "dx": e[1], "dw": e[2]}
+    ms = time_eager(torch, batch.run, K, backlog=True)
+    flops = sum(3 * 2 * c.A.nnz * c.N for c in cases)
+    out = {"ms_per_step": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+           "main_launch_ms": time_eager(torch, lambda: batch.run(stage=2), K, backlog=True),
+           "rel_l2_vs_fp64_oracle": errs}
+    batch.close()
+    return out
 
 
 # --------------------------------------------------------------------------
