@@ -197,6 +197,7 @@ class TrainStep:
             self.dense_flops += 3 * 2 * W.n_rows * W.n_cols * N
             self.alg_bytes += (alg_bytes_fwd(W.n_rows, W.n_cols, W.nnz, N) + alg_bytes_dx(W.n_rows, W.n_cols, W.nnz, N)
                                + alg_bytes_dw(W.n_rows, W.n_cols, W.nnz, N))
+        self.items = items   # per layer: forward, dX, dW (+ update)
         self.batch = T.Batch(items)
 
     def sgd(self, path):
@@ -330,6 +331,14 @@ def bench_train_step(torch, np, T, O, dev, args):
     ts.batch.run(stage=1)
     main_ms = time_eager(torch, lambda: ts.batch.run(stage=2), K, backlog=True)
     prologue_ms = time_eager(torch, lambda: ts.batch.run(stage=1), K, backlog=True)
+    # the persistent launch over one kind of item at a time (the same items:
+    # every layer's forward, dX, or dW with its update), to show its composition
+    main_by_op = {}
+    for k, op in enumerate(("fwd", "dx", "dw")):
+        sub = T.Batch(ts.items[k::3])
+        sub.run(stage=1)
+        main_by_op[op] = time_eager(torch, lambda sub=sub: sub.run(stage=2), K, backlog=True)
+        del sub
     # the separate update kernel the per-call paths launch (the fused path
     # applies it inside the SDDMM units)
     sgd_ms = time_eager(torch, lambda: ts.sgd("f"), K, backlog=True)
@@ -349,7 +358,7 @@ def bench_train_step(torch, np, T, O, dev, args):
     res = {
         "ms_per_step": ms_per_step, "wall_s_timed": wall, "flops_per_step": ts.flops,
         "dense_equivalent_flops_per_step": ts.dense_flops, "alg_bytes_per_step": ts.alg_bytes,
-        "main_launch_ms": main_ms, "prologue_ms": prologue_ms, "sgd_kernel_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
+        "main_launch_ms": main_ms, "main_launch_ms_one_op_at_a_time": main_by_op, "prologue_ms": prologue_ms, "sgd_kernel_ms": sgd_ms, "graph_replay_ms_per_step": graph_ms,
         "cold_ms_per_step": cold_ms, "plain_per_call_cold_ms": plain_cold_ms,
         "plain_per_call_graph_ms": plain_graph, "plain_per_call_eager_ms": plain_eager,
         "dense_per_call_graph_ms": dense_graph, "dense_repack_ms": repack_d,
