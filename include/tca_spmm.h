@@ -308,7 +308,9 @@ tca_status tca_sgd_update(tca_dtype w_dtype, void *d_w, const float *d_g, size_t
 /* An F32 operand (SpMM B, SDDMM G) gets a batch-owned 16-bit copy (2 bytes  */
 /* per element, in the type its readers round it to: FP16 or BF16) that every */
 /* run refreshes first -- same bits as rounding on load -- and the items read */
-/* the copy: half the gather bytes.                                           */
+/* the copy: half the gather bytes.  The copy is per operand: one that the    */
+/* items read with different widths, leading dimensions or 16-bit types, and */
+/* any beyond the 256th, is read as F32 and rounded on load (same results).  */
 /* a_dense = 2 (SPMM, F16/BF16): `a` holds the weight's VALUES, in CSR order */
 /* of the forward matrix (values_order NULL) or in the tiled order of          */
 /* values_order (a forward plan of the same pattern, e.g. the SDDMM item's    */
