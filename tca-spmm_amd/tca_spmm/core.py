@@ -322,6 +322,7 @@ def quantize_rows_e4m3(row_offset, values, stream=None):
 
 def quantize_cols_e4m3(B, stream=None):
     """Per-column E4M3 codes (uint8) and FP32 scales of a dense K x N float16 matrix."""
+    _check_matrix(B, "B", dtypes=(torch.float16,))
     K, N = B.shape
     q = torch.empty((K, N), dtype=torch.uint8, device=B.device)
     scale = torch.empty(N, dtype=torch.float32, device=B.device)

@@ -166,6 +166,15 @@ def test_fp8_quantisers_bit_exact(cuda):
     sbr = np.where(amax > 0, amax / np.float32(448.0), np.float32(1.0)).astype(np.float32)
     assert np.array_equal(sb.cpu().numpy(), sbr)
     assert np.array_equal(qb.cpu().numpy(), O.f32_to_e4m3_fast(Bf / sbr[None, :]))
+    # column ranges of B as views (row pitch 256): offset, odd-width, unaligned start
+    Bd = dev_f16(case.B)
+    for lo, hi in ((8, 136), (0, 250), (3, 131)):
+        qv, sv = T.quantize_cols_e4m3(Bd[:, lo:hi])
+        Bv = Bf[:, lo:hi]
+        amax = np.abs(Bv).max(axis=0)
+        svr = np.where(amax > 0, amax / np.float32(448.0), np.float32(1.0)).astype(np.float32)
+        assert np.array_equal(sv.cpu().numpy(), svr)
+        assert np.array_equal(qv.cpu().numpy(), O.f32_to_e4m3_fast(Bv / svr[None, :]))
 
 
 def test_fp8_spmm_with_scales(cuda):
