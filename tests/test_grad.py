@@ -9,10 +9,10 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import dev, dev_f16, f16_to_f64, rel_l2
+from helpers import dev, dev_bf16, dev_f16, f16_to_f64, rel_l2
 from oracle import oracle as O
 import tca_spmm as T
-from tca_spmm import workloads
+from tca_spmm import gen, workloads
 
 @pytest.fixture(scope="module")
 def problem():
@@ -135,3 +135,30 @@ def test_gpu_gradients_fp32_activations(cuda, problem):
     dW = layer.csr_values(layer.weight.grad).float().cpu().numpy()
     assert rel_l2(dW, ref_dw) <= 1e-3
     assert rel_l2(X.grad.cpu().numpy(), p["dX"]) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_bf16(cuda, problem):
+    """BF16 weights and activations: Y and the FP32 kernel outputs of dW and dX
+    (G rounded to BF16, as the layer rounds it) against the FP64 oracle on the
+    BF16 inputs at 1e-5; W.grad and X.grad, BF16 like their tensors, against the
+    same at 4e-3 (one BF16 rounding of each gradient: ~2^-9 relative)."""
+    p = problem
+    W, M, K, N = p["W"], p["M"], p["K"], p["N"]
+    wb = gen.f32_to_bf16_bits(W.values.view(np.float16).astype(np.float32))
+    xb = gen.f32_to_bf16_bits(p["X"].view(np.float16).astype(np.float32))
+    gb = gen.f32_to_bf16_bits(p["G"])
+    w64, x64, g64 = (O.bf16_bits_to_f32(b).astype(np.float64) for b in (wb, xb, gb))
+    layer = T.SparseLinear(M, K, W.row_offset, W.col_idx, dev_bf16(wb), rows_per_wave=16)
+    X = dev_bf16(xb).requires_grad_(True)
+    Y = layer(X)
+    assert rel_l2(Y.detach().cpu().numpy(), O.spmm_f64(M, N, W.row_offset, W.col_idx, w64, x64)) <= 1e-5
+    (Y * dev(p["G"])).sum().backward()
+    ref_dw = O.sddmm_f64(M, N, W.row_offset, W.col_idx, g64, x64)
+    ref_dx = O.spmm_f64(K, N, p["rpT"], p["cT"], w64[p["perm"]], g64)
+    assert layer.weight.grad.dtype == torch.bfloat16 and X.grad.dtype == torch.bfloat16
+    assert rel_l2(layer.csr_values(layer.weight.grad).float().cpu().numpy(), ref_dw) <= 4e-3
+    assert rel_l2(X.grad.float().cpu().numpy(), ref_dx) <= 4e-3
+    Gb = dev_bf16(gb)
+    assert rel_l2(layer.plan.unpack(T.sddmm(layer.plan, Gb, X.detach())).cpu().numpy(), ref_dw) <= 1e-5
+    assert rel_l2(T.spmm(layer.plan_t, layer.weight.detach(), Gb).cpu().numpy(), ref_dx) <= 1e-5
