@@ -334,7 +334,7 @@ def bench_train_step(torch, np, T, O, dev, args):
     # the persistent launch over one kind of item at a time (the same items:
     # every layer's forward, dX, or dW with its update), to show its composition
     main_by_op = {}
-    for k, op in enumerate(("fwd", "dx", "dw")):
+    for k, op in enumerate(("fwd", "dx", "dw") if "composition" in args.sections else ()):
         sub = T.Batch(ts.items[k::3])
         sub.run(stage=1)
         main_by_op[op] = time_eager(torch, lambda sub=sub: sub.run(stage=2), K, backlog=True)
@@ -840,8 +840,9 @@ def bench_plain_cpu(torch, np, T, O, dev, cores):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="suite,precision,linear,plain_cpu",
-                    help="comma list of extra sections: suite, precision, linear, plain_cpu")
+    ap.add_argument("--sections", default="composition,suite,precision,linear,plain_cpu",
+                    help="comma list of extra sections: composition (the persistent launch over each kind of "
+                         "item alone), suite, precision, linear, plain_cpu")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
